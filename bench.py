@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: DQMC sweeps/s (all chains) of the batched Z2-gauge engine.
+
+One step = one sweep (left + right pass, 4*N*Ltrot link-flip proposals per
+chain) of every chain of the workload, inputs resident in HBM.  Default workload
+(the configuration the metric is quoted on): 64 chains, L=16 (N=256), beta=20,
+Ltrot=200, Nwrap=10, h in {0.4,0.6,0.8,1.0} x 16, J=1, t=1, mu=0.
+
+Prints one JSON line.  `roofline` times the dominant kernel live with HIP events
+on the engine's stream; `cpu_baseline` times the CPU oracle (plain-C FP64
+restatement of the reference path) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "DQMC sweeps/s (all chains) at L=16,β=20: mixed-prec vs FP64; max|ΔG|"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+CONFIGS = {
+    # name: (L, Ltrot, beta, Nwrap, n_chains, h values, mu)
+    "c4": (16, 200, 20.0, 10, 64, (0.4, 0.6, 0.8, 1.0), 0.0),
+    "c2": (12, 120, 12.0, 10, 1, (0.8,), 0.0),
+    "c1": (4, 40, 4.0, 10, 1, (0.5,), 0.0),
+}
+
+
+def chain_specs(name):
+    import z2_gauge_zfy_amd as z
+    L, T, beta, w, nch, hs, mu = CONFIGS[name]
+    per = max(1, nch // len(hs))
+    return [z.ChainSpec(RT=1.0, J=1.0, h=hs[min(k // per, len(hs) - 1)], mu=mu, Beta=beta,
+                        seed=12345 + 2 * 7919 * k, absolute=1) for k in range(nch)]
+
+
+# algorithmic work of one launch of a kernel class, per chain (N sites)
+def kernel_work(kind, N):
+    if kind == "qrp":      # Householder QR with column pivoting: 4/3 N^3 flops
+        return ("mfma", 4.0 / 3.0 * N ** 3)
+    if kind == "orgqr":    # explicit Q: 4/3 N^3 flops
+        return ("mfma", 4.0 / 3.0 * N ** 3)
+    if kind == "gemm":     # N^3 multiply-adds
+        return ("mfma", 2.0 * N ** 3)
+    if kind == "trsm":
+        return ("mfma", 1.0 * N ** 3)
+    if kind in ("kin_rows", "kin_cols"):  # read + write one FP64 N x N matrix
+        return ("hbm", 2.0 * 8 * N * N)
+    if kind == "obs":      # read G twice (G_ij, G_ji)
+        return ("hbm", 2.0 * 8 * N * N)
+    if kind == "flip":     # 2x2 blocks of all 2N bonds + 2 rows/cols per accept (lower bound)
+        return ("hbm", 8.0 * 5 * N)
+    raise KeyError(kind)
+
+
+def _cpu_worker(args):
+    cfg, k, nslices = args
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    L, T, beta, w, nch, hs, mu = CONFIGS[cfg]
+    h = hs[k % len(hs)]
+    p = O.make_params(RT=1.0, J=1.0, h=h, mu=mu, Nlx=L, Nly=L, Ltrot=T, Beta=beta, Nwrap=w)
+    m = O.Model(p)
+    c = O.Chain(m, 12345 + 2 * 7919 * k)
+    c.pre()
+    t0 = time.perf_counter()
+    c.sweep_L_partial(nslices)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(cfg, nslices):
+    L, T, beta, w, nch, hs, mu = CONFIGS[cfg]
+    P = max(1, min(os.cpu_count() or 1, 16, nch))
+    with mp.get_context("spawn").Pool(P) as pool:
+        times = pool.map(_cpu_worker, [(cfg, k, nslices) for k in range(P)])
+    t_sweep = max(times) * (2.0 * T / nslices)  # a sweep = 2*Ltrot slice steps, 2*Ltrot/Nwrap recomputes
+    return {"value": P / t_sweep, "unit": "sweeps/s", "cores": P, "kind": "port",
+            "sample": f"oracle (plain C FP64, 1 thread/process, one chain per process, {P} processes): "
+                      f"first {nslices} slice steps of a left pass incl. {nslices // w} stabilizations "
+                      f"after pre(), extrapolated x{2 * T // nslices} to one sweep; "
+                      f"max over processes {max(times):.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--roofline-kernel", default="qrp")
+    ap.add_argument("--cpu-slices", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE={world}); "
+              "chains are independent, so N GPUs would each run their own chain subset.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import z2_gauge_zfy_amd as z
+    L, T, beta, w, nch, hs, mu = CONFIGS[a.config]
+    N = L * L
+    specs = chain_specs(a.config)
+    eng = z.Engine(L, L, T, w, specs)
+    eng.init_fields()
+    eng.pre()
+    for _ in range(a.warmup):
+        eng.sweep(True)
+    eng.sync()
+    acc0 = sum(eng.counters(k)[0] for k in range(nch))
+    prop0 = sum(eng.counters(k)[1] for k in range(nch))
+    eng.profile_kernel(a.roofline_kernel)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        eng.sweep(True)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms, klaunch = eng.profile_read()
+    eng.profile_kernel("")
+    acc1 = sum(eng.counters(k)[0] for k in range(nch))
+    prop1 = sum(eng.counters(k)[1] for k in range(nch))
+    sweeps_per_s = nch * a.steps / dt
+
+    bound, work = kernel_work(a.roofline_kernel, N)
+    avg_s = kms / 1e3 / max(1, klaunch)
+    per_launch = work * nch
+    if bound == "hbm":
+        achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        achieved, peak, unit = per_launch / avg_s / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
+    out = {
+        "metric": METRIC,
+        "value": sweeps_per_s,
+        "unit": "sweeps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": dt / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random initial links, absolute=1)",
+        "config": {"workload": f"{a.config}: {nch} chains, L={L} (N={N}), beta={beta}, Ltrot={T}, "
+                               f"Nwrap={w}, h in {list(hs)}, J=1, t=1, mu={mu}, FP64",
+                   "n_chains": nch, "L": L, "Ltrot": T, "beta": beta, "Nwrap": w,
+                   "precision": "fp64"},
+        "flip_updates_per_s": (prop1 - prop0) / dt,
+        "accepted_updates_per_s": (acc1 - acc0) / dt,
+        "roofline": {"kernel": a.roofline_kernel, "bound": bound, "achieved": achieved, "peak": peak,
+                     "unit": unit, "frac": achieved / peak, "traffic": None,
+                     "avg_launch_us": avg_s * 1e6, "launches": klaunch,
+                     "kernel_share_of_step": (kms / 1e3) / dt},
+    }
+    eng.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.config, a.cpu_slices)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,125 @@
+/*
+ * z2dqmc.h -- C ABI of the MI355X (gfx950) Z2-gauge DQMC engine.
+ *
+ * Plain C types only: the reference's Fortran modules (or any FFI) bind these
+ * with iso_c_binding / ctypes.  A context holds a batch of independent Markov
+ * chains (one reference MPI rank each) that share the lattice size L, Ltrot and
+ * Nwrap; each chain has its own couplings, seed and field.  All chains of a
+ * context advance together: every entry point below acts on all of them, in
+ * one batched launch per step, on the context's own HIP stream.
+ *
+ * Conventions: every function returns 0 on success and a negative Z2_E* code
+ * on error (z2_last_error() describes it); nothing aborts.  Host buffers are
+ * owned by the caller.  Matrices are N x N column-major (Fortran order) with
+ * 0-based C indexing; sigma is sigma(ii, nt) column-major, ii = 1..2N fastest,
+ * values +-1 as double (reference SigmaConf%sigma, src/fields.f90:10).
+ * Spin-down quantities equal spin-up ones bit for bit in the reference
+ * (identical operations on PropU and PropD), so one Green's function is kept.
+ */
+#ifndef Z2DQMC_H
+#define Z2DQMC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define Z2_OK 0
+#define Z2_EINVAL (-1)   /* bad argument */
+#define Z2_EHIP (-2)     /* HIP runtime error */
+#define Z2_ENOMEM (-3)   /* device allocation failed */
+#define Z2_ESTATE (-4)   /* call out of order (e.g. sweep before pre) */
+
+#define Z2_PREC_FP64 0   /* everything FP64 (the reference's arithmetic) */
+#define Z2_PREC_MIXED 1  /* FP32 propagation/flips/delayed updates, FP64 QR/UDV/G recompute/ratio */
+
+/* shared parameters: the size-defining lines of paramC_sets.txt
+ * (reference src/calc_basic.f90:53-61) plus the precision mode */
+typedef struct {
+  int32_t Nlx, Nly, Ltrot, Nwrap;
+  int32_t Nsweep;     /* sweeps per z2_sweep call (reference Nsweep) */
+  int32_t precision;  /* Z2_PREC_* */
+} z2_params;
+
+/* per-chain couplings (reference RT, J, h, mu, Q, Beta) and RNG seed */
+typedef struct {
+  double RT, J, h, mu, Q, Beta;
+  int32_t seed;
+  int32_t absolute;   /* initial field mode for z2_init_fields (src/fields.f90:322-424) */
+} z2_chain_desc;
+
+typedef struct z2_ctx z2_ctx;
+
+const char *z2_last_error(void);
+
+/* Model_init + Propagator/WrapList make for all chains
+ * (reference src/model.f90:16-36, src/process_matrix.f90:46-56,104-112). */
+int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chain_desc *chains,
+                  z2_ctx **out);
+int z2_ctx_destroy(z2_ctx *ctx);
+int z2_ctx_info(const z2_ctx *ctx, int32_t *N, int32_t *n_chains, int32_t *precision);
+
+/* conf_set (fresh start, confin = 0) with each chain's seed and absolute mode,
+ * run on the host, then uploaded (src/fields.f90:227-309,311-427). */
+int z2_init_fields(z2_ctx *ctx);
+/* field, lambda and seed of one chain (confin restart / conf_out, src/fields.f90:509-557) */
+int z2_set_sigma(z2_ctx *ctx, int32_t chain, const double *sigma, int32_t seed);
+int z2_get_sigma(z2_ctx *ctx, int32_t chain, double *sigma);
+int z2_get_seed(z2_ctx *ctx, int32_t chain, int32_t *seed);
+
+/* LocalSweep%pre (src/local_sweep.f90:87-104) */
+int z2_pre(z2_ctx *ctx);
+/* LocalSweep%sweep (src/local_sweep.f90:152-195): Nsweep x (left + right pass)
+ * with equal-time observables at every slice; is_beta selects the pass order. */
+int z2_sweep(z2_ctx *ctx, int32_t is_beta);
+
+/* One sweep of ctx in lockstep with an FP64 reference context of the same
+ * shape (no counterpart in the reference; measures the precision deviation):
+ * ctx replays ref's accept/reject decisions, and maxdiff[chain] receives
+ * max |G_ctx - G_ref| over the propagated Green's functions just before each
+ * recompute of the sweep. */
+int z2_sweep_shadow(z2_ctx *ctx, z2_ctx *ref, int32_t is_beta, double *maxdiff);
+
+/* fine-grained steps, all chains at slice nt (same semantics as the reference
+ * routines named): */
+int z2_wrap_pre(z2_ctx *ctx, int32_t nt);              /* Wrap_pre   src/stabilization.f90:374-398 */
+int z2_wrap_L(z2_ctx *ctx, int32_t nt, int32_t flag);  /* Wrap_L     src/stabilization.f90:400-437 */
+int z2_wrap_R(z2_ctx *ctx, int32_t nt, int32_t flag);  /* Wrap_R     src/stabilization.f90:439-476 */
+int z2_slice_L(z2_ctx *ctx, int32_t nt);  /* propT_L + LocalK_prop_L  src/multiply.f90:79-93, src/localK.f90:347-373 */
+int z2_slice_R(z2_ctx *ctx, int32_t nt);  /* LocalK_prop_R + propT_R  src/localK.f90:375-398, src/multiply.f90:95-108 */
+int z2_obs_equal_calc(z2_ctx *ctx, int32_t nt);  /* Obs_equal%calc src/obser_equal.f90:73-149 */
+int z2_obs_reset(z2_ctx *ctx);                    /* Obs_equal%reset src/obser_equal.f90:33-48 */
+
+/* results / diagnostics */
+int z2_sync(z2_ctx *ctx);
+int z2_get_green(z2_ctx *ctx, int32_t chain, double *G);
+int z2_set_green(z2_ctx *ctx, int32_t chain, const double *G);
+/* Xmaxm / Xmeanm drift of |G_recomputed - G_propagated| (src/stabilization.f90:427-430) */
+int z2_get_drift(z2_ctx *ctx, int32_t chain, double *xmax, double *xmean);
+/* per-bin averages after z2_sweep: flux_avg, chi_flux, pair_sc_q0, cdw_pi,
+ * cdw_shift, cdw_ratio, density (src/obser_equal.f90:50-71) */
+int z2_get_obs_equal(z2_ctx *ctx, int32_t chain, double out[7]);
+/* cumulative local-flip counts: accepted, proposed */
+int z2_get_counters(z2_ctx *ctx, int32_t chain, uint64_t out[2]);
+/* HIP stream of the context (hipStream_t as void*) for timing by the caller */
+int z2_get_stream(z2_ctx *ctx, void **stream);
+
+/* Kernel timing (no counterpart in the reference; for roofline measurement):
+ * bracket every launch of one kernel class with HIP events on the context's
+ * stream; z2_profile_read sums the elapsed times since z2_profile_kernel. */
+#define Z2_K_NONE 0
+#define Z2_K_QRP 1
+#define Z2_K_FLIP 2
+#define Z2_K_KIN_ROWS 3
+#define Z2_K_KIN_COLS 4
+#define Z2_K_GEMM 5
+#define Z2_K_ORGQR 6
+#define Z2_K_TRSM 7
+#define Z2_K_OBS 8
+int z2_profile_kernel(z2_ctx *ctx, int32_t kind);
+int z2_profile_read(z2_ctx *ctx, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

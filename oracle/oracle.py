@@ -91,6 +91,7 @@ def lib():
             "z2o_pre": (None, [V]),
             "z2o_sweep": (None, [V, C.c_int]),
             "z2o_global_sweep": (None, [V]),
+            "z2o_sweep_L_partial": (None, [V, C.c_int]),
             "z2o_slice_L": (None, [V, C.c_int]),
             "z2o_slice_R": (None, [V, C.c_int]),
             "z2o_get_seed": (I32, [V]),
@@ -261,6 +262,9 @@ class Chain:
 
     def sweep(self, toggle=False):
         lib().z2o_sweep(self.ptr, int(toggle))
+
+    def sweep_L_partial(self, nslices: int):
+        lib().z2o_sweep_L_partial(self.ptr, nslices)
 
     def global_sweep(self):
         lib().z2o_global_sweep(self.ptr)

@@ -1781,6 +1781,19 @@ static void sweep_L(z2o_chain *c, int *Nobs) {
   wrap_L(c, &c->prop, &c->wr, 0, 1);
 }
 
+/* the first nslices slices of a left pass (timing sample for the CPU baseline) */
+void z2o_sweep_L_partial(z2o_chain *c, int nslices) {
+  const int w = c->m->p.Nwrap;
+  int Nobs = 0;
+  for (int nt = c->T; nt >= 1 && nt > c->T - nslices; --nt) {
+    if (nt % w == 0) wrap_L(c, &c->prop, &c->wr, nt, 1);
+    obs_equal_calc(c, nt);
+    Nobs++;
+    propT_L(c, &c->prop);
+    localK_prop_L(c, nt);
+  }
+}
+
 /* Dyn_reset + Dyn_sweep_R, src/dynamics.f90:41-62 */
 static void dyn_sweep(z2o_chain *c) {
   const z2o_model *m = c->m;

@@ -92,6 +92,9 @@ void z2o_therm(z2o_chain *c);                 /* src/local_sweep.f90:73-85 */
 void z2o_pre(z2o_chain *c);                   /* src/local_sweep.f90:87-104 */
 void z2o_sweep(z2o_chain *c, int toggle);     /* src/local_sweep.f90:152-195 */
 void z2o_global_sweep(z2o_chain *c);          /* src/global_update.f90:313-337 */
+/* the first nslices steps of a left pass (src/local_sweep.f90:111-120); a timing
+ * sample for the CPU baseline */
+void z2o_sweep_L_partial(z2o_chain *c, int nslices);
 /* single-slice pieces for fine-grained parity (src/localK.f90:347-398) */
 void z2o_slice_L(z2o_chain *c, int nt);
 void z2o_slice_R(z2o_chain *c, int nt);

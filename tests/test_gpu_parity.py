@@ -1,0 +1,103 @@
+"""GPU engine vs the CPU oracle on the same seeded inputs (FP64 path).
+
+Bar: bit-identical gauge-field history and RNG state (a decision may differ only
+where |ratio - r| < 1e-12, which the oracle's min_gap reports), G within 1e-10
+relative to max|G|, observables within 1e-10 relative."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(L, T, beta, w, specs, Nsweep=1):
+    import z2_gauge_zfy_amd as z
+    return z.Engine(L, L, T, w, specs, Nsweep=Nsweep)
+
+
+def _oracle_chain(L, T, beta, w, spec):
+    p = O.make_params(RT=spec.RT, J=spec.J, h=spec.h, mu=spec.mu, Q=spec.Q, Nlx=L, Nly=L,
+                      Ltrot=T, Beta=beta, Nwrap=w, absolute=spec.absolute)
+    m = O.Model(p)
+    return m, O.Chain(m, spec.seed)
+
+
+def _close_G(Gg, Go, tol=1e-10):
+    scale = max(1.0, np.abs(Go).max())
+    err = np.abs(Gg - Go).max() / scale
+    assert err < tol, err
+
+
+def _run_parity(L, T, beta, w, specs, nsweeps, obs_tol=1e-10):
+    import z2_gauge_zfy_amd as z
+    e = _engine(L, T, beta, w, specs)
+    e.init_fields()
+    chains = [_oracle_chain(L, T, beta, w, s) for s in specs]
+    for k, (m, c) in enumerate(chains):
+        assert np.array_equal(e.sigma(k), c.sigma()), "initial field"
+        assert e.seed(k) == c.seed, "seed after conf_set"
+        c.pre()
+    e.pre()
+    for k, (m, c) in enumerate(chains):
+        _close_G(e.green(k), c.green())
+    for sw in range(nsweeps):
+        e.sweep(True)
+        for k, (m, c) in enumerate(chains):
+            c.sweep()
+            print(f"chain {k} sweep {sw}: min gap {c.min_gap:.3e}", flush=True)
+            same = np.array_equal(e.sigma(k), c.sigma())
+            if not same:
+                assert c.min_gap < 1e-12, f"field history diverged at sweep {sw} (chain {k})"
+                return
+            assert e.seed(k) == c.seed
+            _close_G(e.green(k), c.green())
+            og, oo = e.obs_equal(k), c.obs_equal()
+            for n in ("flux_avg", "chi_flux"):
+                assert og[n] == oo[n], n
+            for n in ("density", "pair_sc_q0", "cdw_pi", "cdw_shift"):
+                assert abs(og[n] - oo[n]) <= obs_tol * max(1.0, abs(oo[n])), n
+            xg, xo = e.drift(k), c.drift()
+            assert abs(xg[0] - xo[0]) <= 1e-9 * max(1e-3, xo[0])
+            acc, prop = e.counters(k)
+            k2 = c.counters()
+            assert acc == k2["local_acc"] and prop == k2["local_prop"]
+
+
+def test_parity_c1_single_chain():
+    """L=4, beta=4, Ltrot=40, h=0.5, Nwrap=10 (the CPU-runnable configuration)."""
+    import z2_gauge_zfy_amd as z
+    _run_parity(4, 40, 4.0, 10, [z.ChainSpec(h=0.5, Beta=4.0, seed=12345)], nsweeps=5)
+
+
+def test_parity_batched_uneven_couplings_and_mu():
+    import z2_gauge_zfy_amd as z
+    specs = [z.ChainSpec(h=0.3 + 0.2 * k, mu=(0.3 if k % 2 else 0.0), J=1.0 + 0.1 * k,
+                         Beta=2.0, seed=12345 + 2 * 7919 * k) for k in range(4)]
+    _run_parity(6, 20, 2.0, 5, specs, nsweeps=3)
+
+
+def test_slice_kernels_bit_exact():
+    """wraps + flips + delayed updates of single slices, no stabilization:
+    bit-identical to the oracle (FP64, no FMA contraction)."""
+    import z2_gauge_zfy_amd as z
+    spec = z.ChainSpec(h=0.4, mu=0.2, Beta=2.0, seed=777)
+    L, T = 6, 20
+    e = _engine(L, T, 2.0, 5, [spec])
+    e.init_fields()
+    m, c = _oracle_chain(L, T, 2.0, 5, spec)
+    rng = np.random.default_rng(0)
+    G = np.asfortranarray(rng.standard_normal((L * L, L * L)) * 0.05 + 0.5 * np.eye(L * L))
+    e.set_green(0, G)
+    c.set_green(G)
+    for nt in (7, 6, 5):
+        e.slice_L(nt)
+        c.slice_L(nt)
+        assert np.array_equal(e.sigma(0), c.sigma())
+        assert np.array_equal(e.green(0), c.green()), f"slice_L {nt}"
+    for nt in (5, 6, 7):
+        e.slice_R(nt)
+        c.slice_R(nt)
+        assert np.array_equal(e.sigma(0), c.sigma())
+        assert np.array_equal(e.green(0), c.green()), f"slice_R {nt}"
+    assert e.seed(0) == c.seed

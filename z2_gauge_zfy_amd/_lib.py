@@ -1,0 +1,110 @@
+"""ctypes loader of the gfx950 engine libz2dqmc.so (C ABI in include/z2dqmc.h).
+
+The library is the product path: when it is missing this module raises
+instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libz2dqmc.so")
+
+Z2_PREC_FP64 = 0
+Z2_PREC_MIXED = 1
+
+# kernel classes for z2_profile_kernel
+KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
+              "obs": 8}
+
+
+class Z2Params(C.Structure):
+    _fields_ = [("Nlx", C.c_int32), ("Nly", C.c_int32), ("Ltrot", C.c_int32), ("Nwrap", C.c_int32),
+                ("Nsweep", C.c_int32), ("precision", C.c_int32)]
+
+
+class Z2ChainDesc(C.Structure):
+    _fields_ = [("RT", C.c_double), ("J", C.c_double), ("h", C.c_double), ("mu", C.c_double),
+                ("Q", C.c_double), ("Beta", C.c_double), ("seed", C.c_int32),
+                ("absolute", C.c_int32)]
+
+
+class Z2Error(RuntimeError):
+    pass
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "z2_last_error": (C.c_char_p, []),
+    "z2_ctx_create": (C.c_int, [C.POINTER(Z2Params), C.c_int32, C.POINTER(Z2ChainDesc),
+                                C.POINTER(C.c_void_p)]),
+    "z2_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "z2_ctx_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                              C.POINTER(C.c_int32)]),
+    "z2_init_fields": (C.c_int, [C.c_void_p]),
+    "z2_set_sigma": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.c_int32]),
+    "z2_get_sigma": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_get_seed": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "z2_pre": (C.c_int, [C.c_void_p]),
+    "z2_sweep": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_sweep_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_wrap_pre": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_wrap_L": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_wrap_R": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_slice_L": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_slice_R": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_obs_equal_calc": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_obs_reset": (C.c_int, [C.c_void_p]),
+    "z2_sync": (C.c_int, [C.c_void_p]),
+    "z2_get_green": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_set_green": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_get_drift": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)]),
+    "z2_get_obs_equal": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    "z2_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    # kernel test hooks (include/z2dqmc_test.h)
+    "z2_test_gemm": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.c_int32,
+                               C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_int32,
+                               C.POINTER(C.c_double)]),
+    "z2_test_qrp": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                              C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                              C.POINTER(C.c_double)]),
+    "z2_test_trsm": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                               C.c_int32]),
+}
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", PKG], check=True)
+
+
+def lib():
+    """Load the engine; raises Z2Error when the HIP library is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise Z2Error(f"{LIB_PATH} is missing: build it with `make -C {PKG}` "
+                          "(the engine has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise Z2Error(f"z2dqmc error {rc}: {lib().z2_last_error().decode()}")
+
+
+def exported_symbols():
+    return list(_SIGS)

@@ -1,0 +1,170 @@
+// Checkerboard kinetic propagator B = E1 E2 E3 E4 E4 E3 E2 E1 applied from the
+// left (row-pair rotations, reference src/operator_kin.f90:61-120) or from the
+// right (column-pair rotations, src/operator_kin.f90:122-179), all 8 stages
+// fused in one pass through LDS.  Optional prologue: scalar mu factors
+// (src/non_interact.f90:34-82) and the delayed rank-2 updates of the flip
+// kernel, applied one after the other in acceptance order -- the same
+// arithmetic as the reference's immediate updates (src/localK.f90:150-152).
+// Optional epilogue: one scalar factor.
+#pragma once
+#include "z2_common.hpp"
+
+// scalar factor selectors
+enum : int { FAC_NONE = 0, FAC_EP = 1, FAC_EM = 2 };
+
+template <typename R>
+struct KinMat {
+  R *mat;         // chain stride N*N
+  int nflag;      // +1 / -1 rotation, 0: none
+  int flush;      // apply delayed updates
+  int pre1, pre2; // factors before the flush
+  int mid1, mid2; // factors after the flush
+  int post;       // factor after the rotation
+};
+
+template <typename R>
+struct KinArgs {
+  KinMat<R> m[2];        // grid.z selects the matrix
+  const int8_t *sig;     // sigma base: chain stride T*2N, slice (nt-1)*2N
+  int T, nt;
+  const ChainConst *cc;  // per chain
+  const R *Tbuf;         // delayed updates: chain stride upd_stride, [(k*2+c)*N + row]
+  const R *Vbuf;         //                                          [(k*2+c)*N + col]
+  const int *nacc;       // accepted updates per chain
+  long upd_stride;
+  LatticeDev lat;
+};
+
+template <typename R>
+__device__ __forceinline__ R apply_fac(R x, int sel, const ChainConst &c) {
+  if (sel == FAC_EP) return x * (R)c.ep;
+  if (sel == FAC_EM) return x * (R)c.em;
+  return x;
+}
+
+// Prologue applied to element (r, col) of a chain's matrix.
+template <typename R>
+__device__ __forceinline__ R kin_prologue(R x, int r, int col, const KinMat<R> &km,
+                                          const KinArgs<R> &a, const ChainConst &c, int chain) {
+  x = apply_fac<R>(x, km.pre1, c);
+  x = apply_fac<R>(x, km.pre2, c);
+  if (km.flush) {
+    const int N = a.lat.N;
+    const int na = a.nacc[chain];
+    const R *T = a.Tbuf + chain * a.upd_stride;
+    const R *V = a.Vbuf + chain * a.upd_stride;
+    for (int k = 0; k < na; ++k) {
+      R t0 = T[(2 * k) * N + r], t1 = T[(2 * k + 1) * N + r];
+      R v0 = V[(2 * k) * N + col], v1 = V[(2 * k + 1) * N + col];
+      R p = t0 * v0;
+      R q = t1 * v1;
+      R d = p + q;
+      x = x - d;
+    }
+  }
+  x = apply_fac<R>(x, km.mid1, c);
+  x = apply_fac<R>(x, km.mid2, c);
+  return x;
+}
+
+// Mat <- B^{nflag} Mat : each column is independent; a block owns CB columns.
+// grid: (ceil(N/CB), n_chains, n_mats)
+template <typename R, int CB>
+__global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
+  extern __shared__ unsigned char smem_raw[];
+  R *tile = reinterpret_cast<R *>(smem_raw);  // [CB][N]
+  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2;
+  const int chain = blockIdx.y;
+  const KinMat<R> km = a.m[blockIdx.z];
+  R *M = km.mat + (size_t)chain * N * N;
+  const int col0 = blockIdx.x * CB;
+  const int ncol = min(CB, N - col0);
+  const ChainConst c = a.cc[chain];
+  for (int idx = threadIdx.x; idx < ncol * N; idx += blockDim.x) {
+    int cc = idx / N, r = idx - cc * N;
+    R x = M[r + (size_t)(col0 + cc) * N];
+    tile[cc * N + r] = kin_prologue<R>(x, r, col0 + cc, km, a, c, chain);
+  }
+  __syncthreads();
+  if (km.nflag != 0) {
+    const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
+    for (int st = 0; st < 8; ++st) {
+      const int *grp = a.lat.grp + kin_group_of_stage(st) * half;
+      for (int idx = threadIdx.x; idx < half * ncol; idx += blockDim.x) {
+        int k = idx % half, cc = idx / half;
+        int b = grp[k];
+        int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
+        int sg = km.nflag * (int)s[b];
+        R C = (R)(sg > 0 ? c.Cp : c.Cm), S = (R)(sg > 0 ? c.Sp : c.Sm);
+        R x1 = tile[cc * N + p1], x2 = tile[cc * N + p2];
+        R v1 = C * x1 + S * x2;
+        R v2 = S * x1 + C * x2;
+        tile[cc * N + p1] = v1;
+        tile[cc * N + p2] = v2;
+      }
+      __syncthreads();
+    }
+  }
+  for (int idx = threadIdx.x; idx < ncol * N; idx += blockDim.x) {
+    int cc = idx / N, r = idx - cc * N;
+    M[r + (size_t)(col0 + cc) * N] = apply_fac<R>(tile[cc * N + r], km.post, c);
+  }
+}
+
+// Mat <- Mat B^{nflag} : each row is independent; a block owns RB rows.
+template <typename R, int RB>
+__global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
+  extern __shared__ unsigned char smem_raw[];
+  R *tile = reinterpret_cast<R *>(smem_raw);  // [N][RB]
+  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2;
+  const int chain = blockIdx.y;
+  const KinMat<R> km = a.m[blockIdx.z];
+  R *M = km.mat + (size_t)chain * N * N;
+  const int row0 = blockIdx.x * RB;
+  const int nrow = min(RB, N - row0);
+  const ChainConst c = a.cc[chain];
+  for (int idx = threadIdx.x; idx < nrow * N; idx += blockDim.x) {
+    int col = idx / nrow, r = idx - col * nrow;
+    R x = M[(row0 + r) + (size_t)col * N];
+    tile[col * RB + r] = kin_prologue<R>(x, row0 + r, col, km, a, c, chain);
+  }
+  __syncthreads();
+  if (km.nflag != 0) {
+    const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
+    for (int st = 0; st < 8; ++st) {
+      const int *grp = a.lat.grp + kin_group_of_stage(st) * half;
+      for (int idx = threadIdx.x; idx < half * nrow; idx += blockDim.x) {
+        int r = idx % nrow, k = idx / nrow;
+        int b = grp[k];
+        int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
+        int sg = km.nflag * (int)s[b];
+        R C = (R)(sg > 0 ? c.Cp : c.Cm), S = (R)(sg > 0 ? c.Sp : c.Sm);
+        R x1 = tile[p1 * RB + r], x2 = tile[p2 * RB + r];
+        R u1 = x1 * C + x2 * S;
+        R u2 = x1 * S + x2 * C;
+        tile[p1 * RB + r] = u1;
+        tile[p2 * RB + r] = u2;
+      }
+      __syncthreads();
+    }
+  }
+  for (int idx = threadIdx.x; idx < nrow * N; idx += blockDim.x) {
+    int col = idx / nrow, r = idx - col * nrow;
+    M[(row0 + r) + (size_t)col * N] = apply_fac<R>(tile[col * RB + r], km.post, c);
+  }
+}
+
+// Elementwise prologue only (flush and/or mu factors) over whole matrices.
+template <typename R>
+__global__ void __launch_bounds__(256) k_kin_elem(KinArgs<R> a) {
+  const int N = a.lat.N;
+  const int chain = blockIdx.y;
+  const KinMat<R> km = a.m[blockIdx.z];
+  R *M = km.mat + (size_t)chain * N * N;
+  const ChainConst c = a.cc[chain];
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N;
+       idx += gridDim.x * blockDim.x) {
+    int col = idx / N, r = idx - col * N;
+    M[idx] = apply_fac<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km.post, c);
+  }
+}

@@ -1,0 +1,423 @@
+// FP64 dense kernels for the UDV stabilization (reference src/stabilization.f90:
+// 42-200): column-pivoted Householder QR (ZGEQP3 semantics, ZLAQP2 column
+// norm downdating), explicit Q (ZUNGQR), batched GEMM on the FP64 matrix cores,
+// triangular solves (ZTRSM) and the permutations (ZLAPMR/ZLAPMT).
+// All matrices N x N column-major, one per chain (stride N*N).
+#pragma once
+#include "z2_common.hpp"
+
+// ------------------------------------------------------------------ QRP
+// One workgroup per matrix, matrix in global memory (L2 resident).
+// out: A holds R (rows normalised by |R_ii|, src/stabilization.f90:61-66) in the
+// upper triangle and the Householder vectors below; tau; ipvt (1-based); D = |R_ii|.
+struct QrpArgs {
+  double *A;
+  double *tau;     // chain stride N
+  int *ipvt;       // chain stride N
+  double *D;       // chain stride N (may alias a propagator D vector)
+  int N;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(1024) k_qrp(QrpArgs a) {
+  const int N = a.N;
+  const int chain = blockIdx.x;
+  double *A = a.A + (size_t)chain * N * N;
+  double *tau = a.tau + (size_t)chain * N;
+  int *ipvt = a.ipvt + (size_t)chain * N;
+  extern __shared__ unsigned char smem_raw[];
+  double *vn1 = reinterpret_cast<double *>(smem_raw);
+  double *vn2 = vn1 + N;
+  double *redv = vn2 + N;                          // 32
+  int *redi = reinterpret_cast<int *>(redv + 32);  // 32
+  __shared__ double s_tau, s_beta, s_scale;
+  __shared__ int s_pvt;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const double tol3z = sqrt(1.1102230246251565e-16);
+
+  // initial column norms
+  for (int j = wid; j < N; j += nw) {
+    const double *col = A + (size_t)j * N;
+    double s = 0.0;
+    for (int r = lane; r < N; r += 64) s += col[r] * col[r];
+    s = wave_sum(s);
+    if (lane == 0) {
+      vn1[j] = sqrt(s);
+      vn2[j] = vn1[j];
+      ipvt[j] = j + 1;
+    }
+  }
+  __syncthreads();
+  for (int i = 0; i < N; ++i) {
+    // pivot: first index of the largest vn1[i..N)
+    double best = -1.0;
+    int bi = N;
+    for (int j = i + threadIdx.x; j < N; j += blockDim.x) {
+      double v = vn1[j];
+      if (v > best) { best = v; bi = j; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      double ov = __shfl_xor(best, off, 64);
+      int oi = __shfl_xor(bi, off, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { redv[wid] = best; redi[wid] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double bv = redv[0];
+      int bj = redi[0];
+      for (int w = 1; w < nw; ++w)
+        if (redv[w] > bv || (redv[w] == bv && redi[w] < bj)) { bv = redv[w]; bj = redi[w]; }
+      s_pvt = bj;
+    }
+    __syncthreads();
+    const int pvt = s_pvt;
+    if (pvt != i) {
+      double *cp = A + (size_t)pvt * N, *ci = A + (size_t)i * N;
+      for (int r = threadIdx.x; r < N; r += blockDim.x) {
+        double t = cp[r];
+        cp[r] = ci[r];
+        ci[r] = t;
+      }
+      if (threadIdx.x == 0) {
+        int it = ipvt[pvt];
+        ipvt[pvt] = ipvt[i];
+        ipvt[i] = it;
+        vn1[pvt] = vn1[i];
+        vn2[pvt] = vn2[i];
+      }
+    }
+    __syncthreads();
+    // Householder reflector for A[i:, i] (dlarfg)
+    double *ci = A + (size_t)i * N;
+    double s = 0.0;
+    for (int r = i + 1 + threadIdx.x; r < N; r += blockDim.x) s += ci[r] * ci[r];
+    s = wave_sum(s);
+    if (lane == 0) redv[wid] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double ss = 0.0;
+      for (int w = 0; w < nw; ++w) ss += redv[w];
+      double xnorm = sqrt(ss);
+      double alpha = ci[i];
+      if (xnorm == 0.0 || i == N - 1) {
+        s_tau = 0.0;
+        s_beta = alpha;
+        s_scale = 1.0;
+      } else {
+        double xa = fabs(alpha), w_ = fmax(xa, xnorm), z_ = fmin(xa, xnorm);
+        double q = z_ / w_;
+        double nrm = w_ * sqrt(1.0 + q * q);
+        double beta = -copysign(nrm, alpha);
+        s_tau = (beta - alpha) / beta;
+        s_scale = 1.0 / (alpha - beta);
+        s_beta = beta;
+      }
+    }
+    __syncthreads();
+    const double tau_i = s_tau, sc = s_scale;
+    for (int r = i + 1 + threadIdx.x; r < N; r += blockDim.x) ci[r] = ci[r] * sc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      ci[i] = s_beta;
+      tau[i] = tau_i;
+    }
+    // apply H(i) to A[i:, i+1:], one wavefront per column; downdate norms
+    for (int j = i + 1 + wid; j < N; j += nw) {
+      double *cj = A + (size_t)j * N;
+      double w = 0.0;
+      if (tau_i != 0.0) {
+        for (int r = i + lane; r < N; r += 64) {
+          double v = (r == i) ? 1.0 : ci[r];
+          w += cj[r] * v;
+        }
+        w = wave_sum(w);
+        double t = -tau_i * w;
+        for (int r = i + lane; r < N; r += 64) {
+          double v = (r == i) ? 1.0 : ci[r];
+          cj[r] = cj[r] + v * t;
+        }
+      }
+      // norm downdate (ZLAQP2)
+      double aij = __shfl(cj[i], 0, 64);  // all lanes read after own writes (lane owning row i wrote it)
+      if (vn1[j] != 0.0) {
+        double rr = fabs(aij) / vn1[j];
+        double temp = 1.0 - rr * rr;
+        temp = temp < 0.0 ? 0.0 : temp;
+        double qq = vn1[j] / vn2[j];
+        double temp2 = temp * qq * qq;
+        if (temp2 <= tol3z) {
+          double s2 = 0.0;
+          for (int r = i + 1 + lane; r < N; r += 64) s2 += cj[r] * cj[r];
+          s2 = wave_sum(s2);
+          if (lane == 0) {
+            vn1[j] = (i < N - 1) ? sqrt(s2) : 0.0;
+            vn2[j] = vn1[j];
+          }
+        } else if (lane == 0) {
+          vn1[j] = vn1[j] * sqrt(temp);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // D = |R_ii|, rows of R normalised
+  for (int r = threadIdx.x; r < N; r += blockDim.x) a.D[(size_t)chain * N + r] = fabs(A[r + (size_t)r * N]);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+    int j = idx / N, r = idx - j * N;
+    if (r <= j) A[idx] = A[idx] / a.D[(size_t)chain * N + r];
+  }
+}
+
+// ------------------------------------------------------------------ ORGQR
+// Q (N x N) from the reflectors (below the diagonal of A) and tau; writes Q to
+// Q (may not alias A).  ZUNG2R order: H(N-1) applied first to the unit matrix.
+__global__ void __launch_bounds__(1024) k_orgqr(const double *Ain, const double *tauin, double *Qout,
+                                                int N) {
+  const int chain = blockIdx.x;
+  const double *A = Ain + (size_t)chain * N * N;
+  const double *tau = tauin + (size_t)chain * N;
+  double *Q = Qout + (size_t)chain * N * N;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+    int j = idx / N, r = idx - j * N;
+    Q[idx] = (r == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int i = N - 1; i >= 0; --i) {
+    const double ti = tau[i];
+    const double *v = A + (size_t)i * N;  // v[i] = 1 implicit, v[r>i] = A[r,i]
+    // columns j > i: Q[i:, j] -= ti * v (v^T Q[i:, j])
+    for (int j = i + 1 + wid; j < N; j += nw) {
+      double *qj = Q + (size_t)j * N;
+      double w = 0.0;
+      for (int r = i + lane; r < N; r += 64) w += qj[r] * ((r == i) ? 1.0 : v[r]);
+      w = wave_sum(w);
+      double t = -ti * w;
+      for (int r = i + lane; r < N; r += 64) qj[r] = qj[r] + ((r == i) ? 1.0 : v[r]) * t;
+    }
+    // column i: (1 - ti) at i, -ti*v below, 0 above
+    double *qi = Q + (size_t)i * N;
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+      if (r < i) qi[r] = 0.0;
+      else if (r == i) qi[r] = 1.0 - ti;
+      else qi[r] = -ti * v[r];
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ GEMM
+// C = op(A) op(B) (+ beta C), batched over chains, FP64 MFMA 16x16x4.
+// trans flags: 0 = N, 1 = T.  triA / triB: use only the upper triangle
+// (row <= col, in stored coordinates) of A / B (ZTRMM as a masked GEMM).
+struct GemmArgs {
+  const double *A, *B;
+  double *C;
+  long sA, sB, sC;  // chain strides
+  int N;
+  int ta, tb, triA, triB;
+  double beta;
+};
+
+#define GT 64   // tile M, N
+#define GK 16   // tile K
+
+__device__ __forceinline__ double gemm_ld(const double *M, int N, int trans, int tri, int r, int c) {
+  // element (r, c) of op(M)
+  int sr = trans ? c : r, sc = trans ? r : c;
+  if (sr >= N || sc >= N) return 0.0;
+  if (tri && sr > sc) return 0.0;
+  return M[sr + (size_t)sc * N];
+}
+
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ double As[GK][GT + 1];  // As[k][m]
+  __shared__ double Bs[GK][GT + 1];  // Bs[k][n]
+  const int N = g.N;
+  const int chain = blockIdx.z;
+  const double *A = g.A + chain * g.sA;
+  const double *B = g.B + chain * g.sB;
+  double *C = g.C + chain * g.sC;
+  const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wm = (wid & 1) * 32, wn = (wid >> 1) * 32;  // each wave: 32x32 = 2x2 MFMA tiles
+  typedef double d4 __attribute__((ext_vector_type(4)));
+  d4 acc[2][2];
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y) acc[x][y] = (d4){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < N; k0 += GK) {
+    for (int idx = threadIdx.x; idx < GT * GK; idx += blockDim.x) {
+      int m = idx % GT, k = idx / GT;
+      As[k][m] = gemm_ld(A, N, g.ta, g.triA, m0 + m, k0 + k);
+      int n = idx % GT, kk = idx / GT;
+      Bs[kk][n] = gemm_ld(B, N, g.tb, g.triB, k0 + kk, n0 + n);
+    }
+    __syncthreads();
+    for (int kk = 0; kk < GK; kk += 4) {
+      const int kl = kk + (lane >> 4), il = lane & 15;
+      double av[2], bv[2];
+      for (int x = 0; x < 2; ++x) av[x] = As[kl][wm + 16 * x + il];
+      for (int y = 0; y < 2; ++y) bv[y] = Bs[kl][wn + 16 * y + il];
+      for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[x], bv[y], acc[x][y], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  for (int x = 0; x < 2; ++x)
+    for (int y = 0; y < 2; ++y)
+      for (int q = 0; q < 4; ++q) {
+        // v_mfma_f64_16x16x4f64 result: lane l, item q holds D[4q + l/16][l%16]
+        int row = m0 + wm + 16 * x + 4 * q + (lane >> 4);
+        int col = n0 + wn + 16 * y + (lane & 15);
+        if (row < N && col < N) {
+          double v = acc[x][y][q];
+          size_t o = row + (size_t)col * N;
+          C[o] = (g.beta != 0.0) ? v + g.beta * C[o] : v;
+        }
+      }
+}
+
+// ------------------------------------------------------------------ TRSM
+// Solve R x = b for each right-hand side, R upper triangular (upper part of
+// Rm incl. diagonal).  Right-hand side c, row i lives at X[i*rs + c*cs].
+// mode 0 (ZTRSM 'L','U','N','N'): x_k = b_k / R_kk, then b_i -= x_k R_ik.
+// mode 1 (ZTRSM 'R','U','C','N' applied to each row): x_k = (1/R_kk) * b_k, then
+//        b_i -= R_ik x_k.
+// A workgroup owns CBT right-hand sides, staged in LDS.
+#define CBT 16
+__global__ void __launch_bounds__(256) k_trsm(const double *Rm, double *X, int N, int rs, int cs,
+                                              int mode) {
+  extern __shared__ unsigned char smem_raw[];
+  double *xb = reinterpret_cast<double *>(smem_raw);  // [CBT][N]
+  double *rk = xb + CBT * N;                          // column k of R
+  const int chain = blockIdx.y;
+  const double *R = Rm + (size_t)chain * N * N;
+  double *Xc = X + (size_t)chain * N * N;
+  const int c0 = blockIdx.x * CBT;
+  const int nc = min(CBT, N - c0);
+  for (int idx = threadIdx.x; idx < nc * N; idx += blockDim.x) {
+    int c = idx / N, i = idx - c * N;
+    xb[c * N + i] = Xc[(size_t)i * rs + (size_t)(c0 + c) * cs];
+  }
+  __syncthreads();
+  for (int k = N - 1; k >= 0; --k) {
+    for (int i = threadIdx.x; i <= k; i += blockDim.x) rk[i] = R[i + (size_t)k * N];
+    __syncthreads();
+    if (threadIdx.x < nc) {
+      double *x = xb + threadIdx.x * N;
+      if (mode == 0) x[k] = x[k] / rk[k];
+      else x[k] = (1.0 / rk[k]) * x[k];
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < nc * k; idx += blockDim.x) {
+      int c = idx / k, i = idx - c * k;
+      double *x = xb + c * N;
+      if (mode == 0) x[i] = x[i] - x[k] * rk[i];
+      else x[i] = x[i] - rk[i] * x[k];
+    }
+    __syncthreads();
+  }
+  for (int idx = threadIdx.x; idx < nc * N; idx += blockDim.x) {
+    int c = idx / N, i = idx - c * N;
+    Xc[(size_t)i * rs + (size_t)(c0 + c) * cs] = xb[c * N + i];
+  }
+}
+
+// ------------------------------------------------------------------ helpers
+// out = op(in) with optional permutation / scaling, one element per thread.
+// mode:
+//  0 copy                      out[r,c] = in[r,c]
+//  1 transpose                 out[r,c] = in[c,r]
+//  2 scale columns by d        out[r,c] = in[r,c] * d[c]
+//  3 transpose, scale cols     out[r,c] = in[c,r] * d[c]
+//  4 rows forward  (ZLAPMR .true.)   out[r,c] = in[p[r]-1, c]
+//  5 cols forward  (ZLAPMT .true.)   out[r,c] = in[r, p[c]-1]
+//  6 rows backward (ZLAPMR .false.)  out[p[r]-1, c] = in[r, c]
+//  7 cols backward (ZLAPMT .false.)  out[r, p[c]-1] = in[r, c]
+//  8 divide rows by d          out[r,c] = in[r,c] / d[r]
+//  9 divide cols by d          out[r,c] = in[r,c] / d[c]
+struct ElemArgs {
+  const double *in;
+  double *out;
+  const double *d;  // chain stride N
+  const int *p;     // chain stride N
+  int N, mode;
+};
+
+__global__ void __launch_bounds__(256) k_elem(ElemArgs e) {
+  const int N = e.N;
+  const int chain = blockIdx.y;
+  const double *in = e.in + (size_t)chain * N * N;
+  double *out = e.out + (size_t)chain * N * N;
+  const double *d = e.d ? e.d + (size_t)chain * N : nullptr;
+  const int *p = e.p ? e.p + (size_t)chain * N : nullptr;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N; idx += gridDim.x * blockDim.x) {
+    int c = idx / N, r = idx - c * N;
+    switch (e.mode) {
+      case 0: out[idx] = in[idx]; break;
+      case 1: out[idx] = in[c + (size_t)r * N]; break;
+      case 2: out[idx] = in[idx] * d[c]; break;
+      case 3: out[idx] = in[c + (size_t)r * N] * d[c]; break;
+      case 4: out[idx] = in[(p[r] - 1) + (size_t)c * N]; break;
+      case 5: out[idx] = in[r + (size_t)(p[c] - 1) * N]; break;
+      case 6: out[(p[r] - 1) + (size_t)c * N] = in[idx]; break;
+      case 7: out[r + (size_t)(p[c] - 1) * N] = in[idx]; break;
+      case 8: out[idx] = in[idx] / d[r]; break;
+      case 9: out[idx] = in[idx] / d[c]; break;
+    }
+  }
+}
+
+// M = X + (DR VRVL DL), transposed if trans (src/stabilization.f90:138-148)
+__global__ void __launch_bounds__(256) k_build_m(const double *X, const double *VRVL, const double *DR,
+                                                 const double *DL, double *M, int N, int trans) {
+  const int chain = blockIdx.y;
+  const double *x = X + (size_t)chain * N * N, *v = VRVL + (size_t)chain * N * N;
+  const double *dr = DR + (size_t)chain * N, *dl = DL + (size_t)chain * N;
+  double *m = M + (size_t)chain * N * N;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N; idx += gridDim.x * blockDim.x) {
+    int c = idx / N, r = idx - c * N;
+    double t = 0.0 + dl[c] * v[idx];
+    double w = 0.0 + dr[r] * t;
+    double val = x[idx] + w;
+    if (trans) m[c + (size_t)r * N] = val;
+    else m[idx] = val;
+  }
+}
+
+// drift = max |Gnew - Gold| per chain; Xmaxm/Xmeanm update; Gold <- Gnew.
+// One block per chain.
+template <typename R>
+__global__ void __launch_bounds__(256) k_drift_replace(const double *Gnew, R *Gold, double *xmax,
+                                                       double *xmean, int N, int accumulate) {
+  const int chain = blockIdx.x;
+  const double *gn = Gnew + (size_t)chain * N * N;
+  R *go = Gold + (size_t)chain * N * N;
+  double m = 0.0;
+  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+    double d = fabs(gn[idx] - (double)go[idx]);
+    m = d > m ? d : m;
+    go[idx] = (R)gn[idx];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  __shared__ double red[16];
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mm = red[w] > mm ? red[w] : mm;
+    if (mm > xmax[chain]) xmax[chain] = mm;
+    if (accumulate) xmean[chain] += mm;
+  }
+}

@@ -1,0 +1,1240 @@
+// Host side of the gfx950 Z2-gauge DQMC engine: context, C ABI (include/z2dqmc.h)
+// and the per-sweep orchestration of the batched kernels.  The order of steps
+// follows the reference drivers LocalSweep_mod / Stabilize_mod / LocalK_mod
+// (src/local_sweep.f90, src/stabilization.f90, src/localK.f90) exactly; every
+// step acts on all chains of the context in one launch.
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/z2dqmc.h"
+#include "flip_kernels.hpp"
+#include "kin_kernels.hpp"
+#include "linalg_kernels.hpp"
+#include "obs_kernels.hpp"
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPC(x)                                                                         \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess)                                                               \
+      return set_err(Z2_EHIP, "%s:%d %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(x)             \
+  do {                     \
+    int r_ = (x);          \
+    if (r_ != Z2_OK) return r_; \
+  } while (0)
+
+extern "C" const char *z2_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------- RNG (host)
+// ran_imada.f:55-68
+static double ranf_host(int32_t *iq) {
+  uint32_t x = (uint32_t)(*iq) * 48828125u;
+  int32_t s = (int32_t)x;
+  if (s < 0) s = (int32_t)((uint32_t)s + 2147483648u);
+  *iq = s;
+  return (double)s / 2147483648.0;
+}
+
+struct z2_ctx {
+  z2_params p;
+  int N = 0, N2 = 0, T = 0, w = 0, Nst = 0, nch = 0, Lx = 0, Ly = 0;
+  hipStream_t stream = nullptr;
+  std::vector<z2_chain_desc> desc;
+  std::vector<ChainConst> hcc;
+  ChainConst *dcc = nullptr;
+  LatticeDev lat{};
+  int *d_int = nullptr;  // all integer lattice tables
+  double *d_cosT = nullptr;
+  uint32_t *d_apow = nullptr;
+  // chain state (FP64)
+  double *G = nullptr, *UUR = nullptr, *VUR = nullptr, *UUL = nullptr, *VUL = nullptr;
+  double *DUR = nullptr, *DUL = nullptr;
+  double *URl = nullptr, *VRl = nullptr, *ULl = nullptr, *VLl = nullptr, *DRl = nullptr,
+         *DLl = nullptr;  // slot-major: [st][chain][.]
+  int8_t *sig = nullptr;
+  int32_t *seed = nullptr;
+  double *Tbuf = nullptr, *Vbuf = nullptr;
+  long upd_stride = 0;
+  int *nacc = nullptr;
+  unsigned long long *accc = nullptr;
+  double *xmax = nullptr, *xmean = nullptr;
+  double *obs_acc = nullptr, *obs_res = nullptr;
+  // scratch
+  double *M = nullptr, *Q = nullptr, *X = nullptr, *Y = nullptr, *Gn = nullptr;
+  double *tau = nullptr, *Dtmp = nullptr;
+  int *ipvt = nullptr;
+  // mixed precision: propagated G, UUR, UUL and the delayed updates in FP32
+  bool mixed = false;
+  float *Gf = nullptr, *UURf = nullptr, *UULf = nullptr, *Tbf = nullptr, *Vbf = nullptr;
+  // decisions of the last flip launch (bit k = proposal k), and decisions to replay
+  uint32_t *rec = nullptr;
+  const uint32_t *force = nullptr;
+  int dwords = 0;
+  double *cmp = nullptr;  // per-chain max |G - G_other| (shadow runs)
+  std::vector<void *> allocs;
+  bool have_fields = false, have_pre = false;
+  // kernel timing hooks: HIP events around every launch of one kernel class
+  int prof_kind = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
+  size_t prof_used = 0;
+};
+
+static int prof_begin(z2_ctx *c, int kind) {
+  if (c->prof_kind != kind) return Z2_OK;
+  if (c->prof_used == c->prof_ev.size()) {
+    hipEvent_t a, b;
+    HIPC(hipEventCreate(&a));
+    HIPC(hipEventCreate(&b));
+    c->prof_ev.push_back({a, b});
+  }
+  HIPC(hipEventRecord(c->prof_ev[c->prof_used].first, c->stream));
+  return Z2_OK;
+}
+
+static int prof_end(z2_ctx *c, int kind) {
+  if (c->prof_kind != kind) return Z2_OK;
+  HIPC(hipEventRecord(c->prof_ev[c->prof_used].second, c->stream));
+  c->prof_used++;
+  return Z2_OK;
+}
+
+template <typename T>
+static int dalloc(z2_ctx *c, T **ptr, size_t n) {
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, sizeof(T) * (n ? n : 1));
+  if (e != hipSuccess) return set_err(Z2_ENOMEM, "hipMalloc(%zu bytes): %s", sizeof(T) * n, hipGetErrorString(e));
+  c->allocs.push_back(p);
+  *ptr = reinterpret_cast<T *>(p);
+  return Z2_OK;
+}
+
+static void free_ctx(z2_ctx *c) {
+  for (auto &e : c->prof_ev) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  for (void *p : c->allocs) (void)hipFree(p);
+  c->allocs.clear();
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// per-chain constants with the reference's libm calls
+static ChainConst make_const(const z2_chain_desc &d, int Ltrot) {
+  ChainConst k;
+  double Dtau = d.Beta / (double)Ltrot;
+  double alpha = 0.5 * Dtau * d.RT;  // src/operator_kin.f90:22-29
+  double alpha_2 = Dtau * d.RT;
+  k.Cp = cosh((1.0 * alpha) * 1.0);
+  k.Cm = cosh((-1.0 * alpha) * 1.0);
+  k.Sp = sinh((1.0 * alpha) * 1.0);
+  k.Sm = sinh((-1.0 * alpha) * 1.0);
+  for (int q = 0; q < 2; ++q) {  // src/operator_kin.f90:40-59
+    double s_old = q == 0 ? 1.0 : -1.0, s_new = -s_old;
+    double C_new = cosh(alpha_2 * s_new), C_old = cosh(alpha_2 * s_old);
+    double S_new = sinh(alpha_2 * s_new), S_old = sinh(alpha_2 * s_old);
+    double d11 = C_new * C_old - S_new * S_old - 1.0;
+    double d12 = S_new * C_old - C_new * S_old;
+    k.delta[q][0] = d11;
+    k.delta[q][1] = d12;
+    k.delta[q][2] = d12;
+    k.delta[q][3] = d11;
+  }
+  // src/fields.f90:52-117
+  double ab = Dtau * d.J, gamma = 0.0;
+  bool temporal = fabs(d.h) > 1.0e-10;
+  if (temporal) {
+    double ta = tanh(Dtau * d.h);
+    if (ta < 1.0e-300) ta = 1.0e-300;
+    gamma = -0.5 * log(ta);
+  }
+  double lub = log(1.0e200);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double dS = 4.0 * (i - 1), dT = 4.0 * (j - 1);
+      double e = ab * dS;
+      if (e > lub) e = lub;
+      if (e < -lub) e = -lub;
+      double r = exp(e);
+      if (temporal) {
+        double e2 = gamma * dT;
+        if (e2 > lub) e2 = lub;
+        if (e2 < -lub) e2 = -lub;
+        r = r * exp(e2);
+      }
+      k.boson[i * 3 + j] = r;
+    }
+  k.ep = exp(Dtau * d.mu);
+  k.em = exp(-Dtau * d.mu);
+  k.beta = d.Beta;
+  return k;
+}
+
+static int npbc(int nr, int L) { return nr < 1 ? nr + L : (nr > L ? nr - L : nr); }
+
+extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chain_desc *chains,
+                             z2_ctx **out) {
+  if (!p || !chains || !out || n_chains <= 0) return set_err(Z2_EINVAL, "null argument or n_chains <= 0");
+  if (p->Nlx < 2 || p->Nly < 2 || (p->Nlx % 2) || (p->Nly % 2))
+    return set_err(Z2_EINVAL, "Nlx, Nly must be even and >= 2 (checkerboard groups are perfect matchings only for even L)");
+  if (p->Ltrot < 1 || p->Nwrap < 1 || p->Ltrot % p->Nwrap)
+    return set_err(Z2_EINVAL, "Ltrot must be a positive multiple of Nwrap");
+  if (p->precision != Z2_PREC_FP64 && p->precision != Z2_PREC_MIXED)
+    return set_err(Z2_EINVAL, "unknown precision mode %d", p->precision);
+  if (p->Nlx * p->Nly > 1024) return set_err(Z2_EINVAL, "N = Nlx*Nly must be <= 1024");
+  z2_ctx *c = new z2_ctx();
+  c->p = *p;
+  c->Lx = p->Nlx;
+  c->Ly = p->Nly;
+  c->N = p->Nlx * p->Nly;
+  c->N2 = 2 * c->N;
+  c->T = p->Ltrot;
+  c->w = p->Nwrap;
+  c->Nst = p->Ltrot / p->Nwrap;
+  c->nch = n_chains;
+  c->desc.assign(chains, chains + n_chains);
+  if (c->p.Nsweep < 1) c->p.Nsweep = 1;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    int r = set_err(Z2_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    delete c;
+    return r;
+  }
+  int rc = Z2_OK;
+  do {
+    const int N = c->N, N2 = c->N2, Lx = c->Lx, Ly = c->Ly;
+    // lattice, src/lattice.f90:25-155 (0-based)
+    auto site = [&](int x, int y) { return (npbc(y, Ly) - 1) * Lx + (npbc(x, Lx) - 1); };
+    std::vector<int> bond_p(2 * N2), grp, bb(6 * N2), stag(N), xy(2 * N), plaq(4 * N);
+    for (int y = 1; y <= Ly; ++y)
+      for (int x = 1; x <= Lx; ++x) {
+        int s = site(x, y);
+        bond_p[2 * (2 * s)] = s;
+        bond_p[2 * (2 * s) + 1] = site(x + 1, y);
+        bond_p[2 * (2 * s + 1)] = s;
+        bond_p[2 * (2 * s + 1) + 1] = site(x, y + 1);
+        xy[2 * s] = x;
+        xy[2 * s + 1] = y;
+        stag[s] = ((x + y) % 2 == 0) ? 1 : -1;
+      }
+    auto bx = [&](int x, int y) { return 2 * site(x, y); };      // +x bond from (x,y)
+    auto by = [&](int x, int y) { return 2 * site(x, y) + 1; };  // +y bond from (x,y)
+    std::vector<int> g4[4];
+    for (int b = 0; b < N2; ++b) {
+      int s = b / 2, x = xy[2 * s], y = xy[2 * s + 1];
+      if (b % 2 == 0) g4[(x % 2 == 0) ? 0 : 1].push_back(b);
+      else g4[(y % 2 == 0) ? 2 : 3].push_back(b);
+      int *q = &bb[6 * b];
+      if (b % 2 == 0) {  // horizontal, src/lattice.f90:280-292
+        q[0] = by(x, y); q[1] = bx(x, y + 1); q[2] = by(x + 1, y);
+        q[3] = by(x, y - 1); q[4] = bx(x, y - 1); q[5] = by(x + 1, y - 1);
+      } else {           // vertical, src/lattice.f90:293-305
+        q[0] = bx(x - 1, y + 1); q[1] = by(x - 1, y); q[2] = bx(x - 1, y);
+        q[3] = bx(x, y + 1); q[4] = by(x + 1, y); q[5] = bx(x, y);
+      }
+      if (b % 2 == 0) {  // plaquette with this bond at the bottom: bottom,right,top,left
+        int pl = 4 * s;
+        plaq[pl] = bx(x, y); plaq[pl + 1] = by(x + 1, y); plaq[pl + 2] = bx(x, y + 1); plaq[pl + 3] = by(x, y);
+      }
+    }
+    for (int g = 0; g < 4; ++g) {
+      if ((int)g4[g].size() != N / 2) { rc = set_err(Z2_EINVAL, "checkerboard group size"); break; }
+      grp.insert(grp.end(), g4[g].begin(), g4[g].end());
+    }
+    if (rc) break;
+    std::vector<int> all;
+    size_t off_bond = all.size(); all.insert(all.end(), bond_p.begin(), bond_p.end());
+    size_t off_grp = all.size(); all.insert(all.end(), grp.begin(), grp.end());
+    size_t off_bb = all.size(); all.insert(all.end(), bb.begin(), bb.end());
+    size_t off_stag = all.size(); all.insert(all.end(), stag.begin(), stag.end());
+    size_t off_xy = all.size(); all.insert(all.end(), xy.begin(), xy.end());
+    size_t off_plaq = all.size(); all.insert(all.end(), plaq.begin(), plaq.end());
+    if ((rc = dalloc(c, &c->d_int, all.size()))) break;
+    if (hipMemcpy(c->d_int, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "copy tables"); break; }
+    c->lat.N = N;
+    c->lat.N2 = N2;
+    c->lat.bond_p = c->d_int + off_bond;
+    c->lat.grp = c->d_int + off_grp;
+    c->lat.bond_bonds = c->d_int + off_bb;
+    c->lat.stag = c->d_int + off_stag;
+    c->lat.xy = c->d_int + off_xy;
+    c->lat.plaq = c->d_int + off_plaq;
+    // cos table, src/obser_equal.f90:92,139
+    const double PI = acos(-1.0);
+    double qx = PI - 2.0 * PI / (double)Lx;
+    std::vector<double> cosT((2 * Lx - 1) * (2 * Ly - 1));
+    for (int dy = -(Ly - 1); dy <= Ly - 1; ++dy)
+      for (int dx = -(Lx - 1); dx <= Lx - 1; ++dx)
+        cosT[(dx + Lx - 1) + (2 * Lx - 1) * (dy + Ly - 1)] = cos(qx * (double)dx + PI * (double)dy);
+    if ((rc = dalloc(c, &c->d_cosT, cosT.size()))) break;
+    if (hipMemcpy(c->d_cosT, cosT.data(), sizeof(double) * cosT.size(), hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "copy cos"); break; }
+    std::vector<uint32_t> apow(N2 + 1);
+    uint64_t a = 1;
+    for (int k = 0; k <= N2; ++k) { apow[k] = (uint32_t)a; a = (a * 48828125ull) & 0x7fffffffull; }
+    if ((rc = dalloc(c, &c->d_apow, apow.size()))) break;
+    if (hipMemcpy(c->d_apow, apow.data(), sizeof(uint32_t) * apow.size(), hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "copy apow"); break; }
+    for (int i = 0; i < n_chains; ++i) c->hcc.push_back(make_const(chains[i], c->T));
+    if ((rc = dalloc(c, &c->dcc, n_chains))) break;
+    if (hipMemcpy(c->dcc, c->hcc.data(), sizeof(ChainConst) * n_chains, hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "copy consts"); break; }
+    // state
+    const size_t nn = (size_t)N * N, B = (size_t)n_chains;
+    const size_t L = (size_t)(c->Nst + 1);
+    if ((rc = dalloc(c, &c->G, B * nn))) break;
+    if ((rc = dalloc(c, &c->UUR, B * nn))) break;
+    if ((rc = dalloc(c, &c->VUR, B * nn))) break;
+    if ((rc = dalloc(c, &c->UUL, B * nn))) break;
+    if ((rc = dalloc(c, &c->VUL, B * nn))) break;
+    if ((rc = dalloc(c, &c->DUR, B * N))) break;
+    if ((rc = dalloc(c, &c->DUL, B * N))) break;
+    if ((rc = dalloc(c, &c->URl, L * B * nn))) break;
+    if ((rc = dalloc(c, &c->VRl, L * B * nn))) break;
+    if ((rc = dalloc(c, &c->ULl, L * B * nn))) break;
+    if ((rc = dalloc(c, &c->VLl, L * B * nn))) break;
+    if ((rc = dalloc(c, &c->DRl, L * B * N))) break;
+    if ((rc = dalloc(c, &c->DLl, L * B * N))) break;
+    if ((rc = dalloc(c, &c->sig, B * c->T * N2))) break;
+    if ((rc = dalloc(c, &c->seed, B))) break;
+    c->upd_stride = (long)4 * N * N2;  // up to 2N accepts x (2 vectors of N)
+    if (p->precision == Z2_PREC_FP64) {
+      if ((rc = dalloc(c, &c->Tbuf, B * c->upd_stride))) break;
+      if ((rc = dalloc(c, &c->Vbuf, B * c->upd_stride))) break;
+    }
+    if ((rc = dalloc(c, &c->nacc, B))) break;
+    if ((rc = dalloc(c, &c->accc, 2 * B))) break;
+    if ((rc = dalloc(c, &c->xmax, B))) break;
+    if ((rc = dalloc(c, &c->xmean, B))) break;
+    if ((rc = dalloc(c, &c->obs_acc, B * OBS_NACC))) break;
+    if ((rc = dalloc(c, &c->obs_res, B * RES_N))) break;
+    if ((rc = dalloc(c, &c->M, B * nn))) break;
+    if ((rc = dalloc(c, &c->Q, B * nn))) break;
+    if ((rc = dalloc(c, &c->X, B * nn))) break;
+    if ((rc = dalloc(c, &c->Y, B * nn))) break;
+    if ((rc = dalloc(c, &c->Gn, B * nn))) break;
+    if ((rc = dalloc(c, &c->tau, B * N))) break;
+    if ((rc = dalloc(c, &c->Dtmp, B * N))) break;
+    if ((rc = dalloc(c, &c->ipvt, B * N))) break;
+    c->dwords = (N2 + 31) / 32;
+    if ((rc = dalloc(c, &c->rec, B * c->dwords))) break;
+    if ((rc = dalloc(c, &c->cmp, B))) break;
+    c->mixed = p->precision == Z2_PREC_MIXED;
+    if (c->mixed) {
+      if ((rc = dalloc(c, &c->Gf, B * nn))) break;
+      if ((rc = dalloc(c, &c->UURf, B * nn))) break;
+      if ((rc = dalloc(c, &c->UULf, B * nn))) break;
+      if ((rc = dalloc(c, &c->Tbf, B * c->upd_stride))) break;
+      if ((rc = dalloc(c, &c->Vbf, B * c->upd_stride))) break;
+    }
+    // Propagator%make / WrapList%make: identities, zero lists
+    std::vector<double> I(B * nn, 0.0), ones(B * N, 1.0);
+    for (size_t b = 0; b < B; ++b)
+      for (int i = 0; i < N; ++i) I[b * nn + i + (size_t)i * N] = 1.0;
+    for (double *m : {c->G, c->UUR, c->VUR, c->UUL, c->VUL})
+      if (hipMemcpy(m, I.data(), sizeof(double) * B * nn, hipMemcpyHostToDevice) != hipSuccess) rc = Z2_EHIP;
+    for (double *d : {c->DUR, c->DUL})
+      if (hipMemcpy(d, ones.data(), sizeof(double) * B * N, hipMemcpyHostToDevice) != hipSuccess) rc = Z2_EHIP;
+    if (rc) { rc = set_err(Z2_EHIP, "init copies"); break; }
+    if (hipMemset(c->URl, 0, sizeof(double) * L * B * nn) || hipMemset(c->VRl, 0, sizeof(double) * L * B * nn) ||
+        hipMemset(c->ULl, 0, sizeof(double) * L * B * nn) || hipMemset(c->VLl, 0, sizeof(double) * L * B * nn) ||
+        hipMemset(c->DRl, 0, sizeof(double) * L * B * N) || hipMemset(c->DLl, 0, sizeof(double) * L * B * N) ||
+        hipMemset(c->accc, 0, sizeof(unsigned long long) * 2 * B) || hipMemset(c->xmax, 0, sizeof(double) * B) ||
+        hipMemset(c->xmean, 0, sizeof(double) * B) || hipMemset(c->obs_acc, 0, sizeof(double) * B * OBS_NACC) ||
+        hipMemset(c->obs_res, 0, sizeof(double) * B * RES_N) || hipMemset(c->nacc, 0, sizeof(int) * B)) {
+      rc = set_err(Z2_EHIP, "memset");
+      break;
+    }
+    if (c->mixed) {
+      std::vector<float> If(I.begin(), I.end());
+      for (float *m : {c->Gf, c->UURf, c->UULf})
+        if (hipMemcpy(m, If.data(), sizeof(float) * B * nn, hipMemcpyHostToDevice) != hipSuccess) rc = Z2_EHIP;
+      if (rc) { rc = set_err(Z2_EHIP, "init copies (fp32)"); break; }
+    }
+    std::vector<int32_t> seeds(B);
+    for (size_t b = 0; b < B; ++b) seeds[b] = chains[b].seed;
+    if (hipMemcpy(c->seed, seeds.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "seed"); break; }
+  } while (0);
+  if (rc) {
+    free_ctx(c);
+    return rc;
+  }
+  *out = c;
+  return Z2_OK;
+}
+
+extern "C" int z2_ctx_destroy(z2_ctx *c) {
+  if (!c) return Z2_OK;
+  (void)hipStreamSynchronize(c->stream);
+  free_ctx(c);
+  return Z2_OK;
+}
+
+extern "C" int z2_ctx_info(const z2_ctx *c, int32_t *N, int32_t *nch, int32_t *prec) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (N) *N = c->N;
+  if (nch) *nch = c->nch;
+  if (prec) *prec = c->p.precision;
+  return Z2_OK;
+}
+
+extern "C" int z2_get_stream(z2_ctx *c, void **s) {
+  if (!c || !s) return set_err(Z2_EINVAL, "null argument");
+  *s = (void *)c->stream;
+  return Z2_OK;
+}
+
+extern "C" int z2_sync(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- fields
+// conf_set, src/fields.f90:311-427 (therm size == run size: conf_transfer is the identity)
+static void conf_set_host(int Lx, int Ly, int T, int absolute, int32_t *seed, std::vector<double> &sig) {
+  const int N = Lx * Ly, N2 = 2 * N;
+  sig.assign((size_t)N2 * T, 1.0);
+  if (absolute == 1) {
+    for (int nt = 0; nt < T; ++nt)
+      for (int ii = 0; ii < N2; ++ii) sig[(size_t)nt * N2 + ii] = ranf_host(seed) >= 0.5 ? 1.0 : -1.0;
+  } else if (absolute == 2) {
+    const double PI = acos(-1.0);
+    for (int nt = 0; nt < T; ++nt)
+      for (int ii = 0; ii < N2; ++ii) {
+        double X1 = ranf_host(seed), X2 = ranf_host(seed);
+        double X = sqrt(-2.0 * log(X1)) * cos(2.0 * PI * X2);
+        sig[(size_t)nt * N2 + ii] = X >= 0.0 ? 1.0 : -1.0;
+      }
+  } else if (absolute == 4) {
+    for (int nt = 0; nt < T; ++nt)
+      for (int nc = 0; nc < N2; ++nc) {
+        int s = nc / 2, x = s % Lx + 1, y = s / Lx + 1;
+        sig[(size_t)nt * N2 + nc] = ((x + y) % 2 == 0) ? 1.0 : -1.0;
+      }
+  } else if (absolute == 5) {
+    for (int ii = 0; ii < N2; ++ii) {
+      double v = ranf_host(seed) >= 0.5 ? 1.0 : -1.0;
+      for (int nt = 0; nt < T; ++nt) sig[(size_t)nt * N2 + ii] = v;
+    }
+  }
+  for (int i = 0; i < N; ++i) (void)ranf_host(seed);  // lambda_random_fill draws, src/fields.f90:559-578
+}
+
+static int upload_sigma(z2_ctx *c, int chain, const double *sigma) {
+  std::vector<int8_t> s((size_t)c->T * c->N2);
+  for (size_t k = 0; k < s.size(); ++k) {
+    if (sigma[k] != 1.0 && sigma[k] != -1.0) return set_err(Z2_EINVAL, "sigma values must be +-1");
+    s[k] = sigma[k] > 0 ? 1 : -1;
+  }
+  HIPC(hipMemcpyAsync(c->sig + (size_t)chain * s.size(), s.data(), s.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_init_fields(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  std::vector<double> sig;
+  std::vector<int32_t> seeds(c->nch);
+  for (int b = 0; b < c->nch; ++b) {
+    int32_t s = c->desc[b].seed;
+    int ab = c->desc[b].absolute;
+    if (ab < 1 || ab > 5) return set_err(Z2_EINVAL, "absolute must be 1..5");
+    conf_set_host(c->Lx, c->Ly, c->T, ab, &s, sig);
+    CHK(upload_sigma(c, b, sig.data()));
+    seeds[b] = s;
+  }
+  HIPC(hipMemcpy(c->seed, seeds.data(), sizeof(int32_t) * c->nch, hipMemcpyHostToDevice));
+  c->have_fields = true;
+  return Z2_OK;
+}
+
+extern "C" int z2_set_sigma(z2_ctx *c, int32_t chain, const double *sigma, int32_t seed) {
+  if (!c || !sigma || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  CHK(upload_sigma(c, chain, sigma));
+  HIPC(hipMemcpy(c->seed + chain, &seed, sizeof(int32_t), hipMemcpyHostToDevice));
+  c->have_fields = true;
+  return Z2_OK;
+}
+
+extern "C" int z2_get_sigma(z2_ctx *c, int32_t chain, double *sigma) {
+  if (!c || !sigma || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  std::vector<int8_t> s((size_t)c->T * c->N2);
+  HIPC(hipMemcpyAsync(s.data(), c->sig + (size_t)chain * s.size(), s.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < s.size(); ++k) sigma[k] = (double)s[k];
+  return Z2_OK;
+}
+
+extern "C" int z2_get_seed(z2_ctx *c, int32_t chain, int32_t *seed) {
+  if (!c || !seed || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  HIPC(hipMemcpyAsync(seed, c->seed + chain, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+static int get_mat(z2_ctx *c, const double *base, int chain, double *out) {
+  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  size_t nn = (size_t)c->N * c->N;
+  HIPC(hipMemcpyAsync(out, base + chain * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_get_green(z2_ctx *c, int32_t chain, double *G) {
+  if (c && c->mixed) {
+    if (!G || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+    size_t nn = (size_t)c->N * c->N;
+    std::vector<float> f(nn);
+    HIPC(hipMemcpyAsync(f.data(), c->Gf + chain * nn, sizeof(float) * nn, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < nn; ++k) G[k] = (double)f[k];
+    return Z2_OK;
+  }
+  return get_mat(c, c ? c->G : nullptr, chain, G);
+}
+
+extern "C" int z2_set_green(z2_ctx *c, int32_t chain, const double *G) {
+  if (!c || !G || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  size_t nn = (size_t)c->N * c->N;
+  if (c->mixed) {
+    std::vector<float> f(G, G + nn);
+    HIPC(hipMemcpyAsync(c->Gf + chain * nn, f.data(), sizeof(float) * nn, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPC(hipMemcpyAsync(c->G + chain * nn, G, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_get_drift(z2_ctx *c, int32_t chain, double *xmax, double *xmean) {
+  if (!c || !xmax || !xmean || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  HIPC(hipMemcpyAsync(xmax, c->xmax + chain, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(xmean, c->xmean + chain, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_get_obs_equal(z2_ctx *c, int32_t chain, double out[7]) {
+  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  HIPC(hipMemcpyAsync(out, c->obs_res + (size_t)chain * RES_N, sizeof(double) * RES_N, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_get_counters(z2_ctx *c, int32_t chain, uint64_t out[2]) {
+  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  unsigned long long v[2];
+  HIPC(hipMemcpyAsync(v, c->accc + 2 * chain, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  out[0] = v[0];
+  out[1] = v[1];
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- launch helpers
+#define LAUNCH_CHECK()                                                            \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess) return set_err(Z2_EHIP, "launch %s:%d: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+  } while (0)
+
+// buffers of the propagated quantities in precision R
+template <typename R> struct PB;
+template <> struct PB<double> {
+  static double *G(z2_ctx *c) { return c->G; }
+  static double *UUR(z2_ctx *c) { return c->UUR; }
+  static double *UUL(z2_ctx *c) { return c->UUL; }
+  static double *Tb(z2_ctx *c) { return c->Tbuf; }
+  static double *Vb(z2_ctx *c) { return c->Vbuf; }
+};
+template <> struct PB<float> {
+  static float *G(z2_ctx *c) { return c->Gf; }
+  static float *UUR(z2_ctx *c) { return c->UURf; }
+  static float *UUL(z2_ctx *c) { return c->UULf; }
+  static float *Tb(z2_ctx *c) { return c->Tbf; }
+  static float *Vb(z2_ctx *c) { return c->Vbf; }
+};
+
+template <typename R>
+static KinArgs<R> kin_args(z2_ctx *c, int nt) {
+  KinArgs<R> a;
+  memset(&a, 0, sizeof(a));
+  a.sig = c->sig;
+  a.T = c->T;
+  a.nt = nt;
+  a.cc = c->dcc;
+  a.Tbuf = PB<R>::Tb(c);
+  a.Vbuf = PB<R>::Vb(c);
+  a.nacc = c->nacc;
+  a.upd_stride = c->upd_stride;
+  a.lat = c->lat;
+  return a;
+}
+
+template <typename R>
+static KinMat<R> kmat(R *m, int nflag, int flush = 0, int pre1 = 0, int pre2 = 0, int mid1 = 0,
+                      int mid2 = 0, int post = 0) {
+  KinMat<R> k;
+  k.mat = m;
+  k.nflag = nflag;
+  k.flush = flush;
+  k.pre1 = pre1;
+  k.pre2 = pre2;
+  k.mid1 = mid1;
+  k.mid2 = mid2;
+  k.post = post;
+  return k;
+}
+
+static const int KCB = 16;  // columns per block, row-rotation kernel
+static const int KRB = 16;  // rows per block, column-rotation kernel
+
+template <typename R>
+static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  dim3 grid((c->N + KCB - 1) / KCB, c->nch, nmat);
+  size_t sh = sizeof(R) * KCB * c->N;
+  CHK(prof_begin(c, Z2_K_KIN_ROWS));
+  hipLaunchKernelGGL((k_kin_rows<R, KCB>), grid, dim3(256), sh, c->stream, a);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_KIN_ROWS));
+  return Z2_OK;
+}
+
+template <typename R>
+static int launch_cols(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  dim3 grid((c->N + KRB - 1) / KRB, c->nch, nmat);
+  size_t sh = sizeof(R) * KRB * c->N;
+  CHK(prof_begin(c, Z2_K_KIN_COLS));
+  hipLaunchKernelGGL((k_kin_cols<R, KRB>), grid, dim3(256), sh, c->stream, a);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_KIN_COLS));
+  return Z2_OK;
+}
+
+template <typename R>
+static int launch_elem_kin(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  int nb = (c->N * c->N + 255) / 256;
+  if (nb > 64) nb = 64;
+  dim3 grid(nb, c->nch, nmat);
+  hipLaunchKernelGGL((k_kin_elem<R>), grid, dim3(256), 0, c->stream, a);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+template <typename R>
+static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
+  FlipArgs<R> f;
+  f.G = PB<R>::G(c);
+  f.sig = c->sig;
+  f.seed = c->seed;
+  f.Tbuf = PB<R>::Tb(c);
+  f.Vbuf = PB<R>::Vb(c);
+  f.nacc = c->nacc;
+  f.acc_count = c->accc;
+  f.cc = c->dcc;
+  f.apow = c->d_apow;
+  f.upd_stride = c->upd_stride;
+  f.T = c->T;
+  f.nt = nt;
+  f.descending = descending;
+  f.mu_on_read = mu_on_read;
+  f.force = c->force;
+  f.record = c->rec;
+  f.dwords = c->dwords;
+  f.lat = c->lat;
+  size_t sh = sizeof(double) * (c->N + 5 * c->N2) + sizeof(int) * c->N2;
+  CHK(prof_begin(c, Z2_K_FLIP));
+  hipLaunchKernelGGL((k_flip<R>), dim3(c->nch), dim3(256), sh, c->stream, f);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_FLIP));
+  return Z2_OK;
+}
+
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(256) k_cvt(const Tin *in, Tout *out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (Tout)in[i];
+}
+
+template <typename Tin, typename Tout>
+static int cvt(z2_ctx *c, const Tin *in, Tout *out) {
+  size_t n = (size_t)c->nch * c->N * c->N;
+  int nb = (int)((n + 255) / 256);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL((k_cvt<Tin, Tout>), dim3(nb), dim3(256), 0, c->stream, in, out, n);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+// per-chain max |A - B| into out (max with the value already there)
+template <typename Ra, typename Rb>
+__global__ void __launch_bounds__(256) k_maxdiff(const Ra *A, const Rb *B, double *out, int N) {
+  const int chain = blockIdx.x;
+  const Ra *a = A + (size_t)chain * N * N;
+  const Rb *b = B + (size_t)chain * N * N;
+  double m = 0.0;
+  for (int i = threadIdx.x; i < N * N; i += blockDim.x) {
+    double d = fabs((double)a[i] - (double)b[i]);
+    m = d > m ? d : m;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = out[chain];
+    for (int w = 0; w < 4; ++w) mm = red[w] > mm ? red[w] : mm;
+    out[chain] = mm;
+  }
+}
+
+static int elem(z2_ctx *c, const double *in, double *out, int mode, const double *d = nullptr,
+                const int *p = nullptr) {
+  ElemArgs e;
+  e.in = in;
+  e.out = out;
+  e.d = d;
+  e.p = p;
+  e.N = c->N;
+  e.mode = mode;
+  int nb = (c->N * c->N + 255) / 256;
+  if (nb > 64) nb = 64;
+  hipLaunchKernelGGL(k_elem, dim3(nb, c->nch), dim3(256), 0, c->stream, e);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+static int gemm(z2_ctx *c, const double *A, int ta, const double *B, int tb, double *C, int triA = 0,
+                int triB = 0) {
+  GemmArgs g;
+  g.A = A;
+  g.B = B;
+  g.C = C;
+  g.sA = g.sB = g.sC = (long)c->N * c->N;
+  g.N = c->N;
+  g.ta = ta;
+  g.tb = tb;
+  g.triA = triA;
+  g.triB = triB;
+  g.beta = 0.0;
+  dim3 grid((c->N + GT - 1) / GT, (c->N + GT - 1) / GT, c->nch);
+  CHK(prof_begin(c, Z2_K_GEMM));
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, c->stream, g);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_GEMM));
+  return Z2_OK;
+}
+
+static int qrp(z2_ctx *c, double *A, double *D) {
+  QrpArgs q;
+  q.A = A;
+  q.tau = c->tau;
+  q.ipvt = c->ipvt;
+  q.D = D;
+  q.N = c->N;
+  size_t sh = sizeof(double) * (2 * c->N + 32) + sizeof(int) * 32;
+  CHK(prof_begin(c, Z2_K_QRP));
+  hipLaunchKernelGGL(k_qrp, dim3(c->nch), dim3(1024), sh, c->stream, q);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_QRP));
+  return Z2_OK;
+}
+
+static int orgqr(z2_ctx *c, const double *A, double *Qo) {
+  CHK(prof_begin(c, Z2_K_ORGQR));
+  hipLaunchKernelGGL(k_orgqr, dim3(c->nch), dim3(1024), 0, c->stream, A, (const double *)c->tau, Qo, c->N);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_ORGQR));
+  return Z2_OK;
+}
+
+static int trsm(z2_ctx *c, const double *R, double *X, int right) {
+  dim3 grid((c->N + CBT - 1) / CBT, c->nch);
+  size_t sh = sizeof(double) * (CBT + 1) * c->N;
+  CHK(prof_begin(c, Z2_K_TRSM));
+  if (right) hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, c->stream, R, X, c->N, c->N, 1, 1);
+  else hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, c->stream, R, X, c->N, 1, c->N, 0);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_TRSM));
+  return Z2_OK;
+}
+
+static int dcopy(z2_ctx *c, double *dst, const double *src, size_t n) {
+  HIPC(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- stabilization
+// stab_UR, src/stabilization.f90:70-94
+static int stab_UR(z2_ctx *c) {
+  CHK(elem(c, c->UUR, c->M, 2, c->DUR));
+  CHK(qrp(c, c->M, c->DUR));
+  CHK(elem(c, c->VUR, c->X, 4, nullptr, c->ipvt));
+  CHK(gemm(c, c->M, 0, c->X, 0, c->VUR, 1, 0));
+  CHK(orgqr(c, c->M, c->UUR));
+  return Z2_OK;
+}
+
+// stab_UL, src/stabilization.f90:96-120
+static int stab_UL(z2_ctx *c) {
+  CHK(elem(c, c->UUL, c->M, 3, c->DUL));
+  CHK(qrp(c, c->M, c->DUL));
+  CHK(elem(c, c->VUL, c->X, 5, nullptr, c->ipvt));
+  CHK(gemm(c, c->X, 0, c->M, 1, c->VUL, 0, 1));
+  CHK(orgqr(c, c->M, c->Q));
+  CHK(elem(c, c->Q, c->UUL, 1));
+  return Z2_OK;
+}
+
+// stab_green, src/stabilization.f90:122-200 -> c->Gn
+static int stab_green(z2_ctx *c, int nt) {
+  CHK(gemm(c, c->VUR, 0, c->VUL, 0, c->Y));
+  CHK(gemm(c, c->UUR, 1, c->UUL, 1, c->X));
+  const int trans = nt > c->T / 2;
+  {
+    int nb = (c->N * c->N + 255) / 256;
+    if (nb > 64) nb = 64;
+    hipLaunchKernelGGL(k_build_m, dim3(nb, c->nch), dim3(256), 0, c->stream, (const double *)c->X,
+                       (const double *)c->Y, (const double *)c->DUR, (const double *)c->DUL, c->M, c->N, trans);
+    LAUNCH_CHECK();
+  }
+  CHK(qrp(c, c->M, c->Dtmp));
+  CHK(orgqr(c, c->M, c->Q));
+  if (nt < c->T / 2 + 1) {
+    CHK(gemm(c, c->Q, 1, c->UUR, 1, c->X));
+    CHK(elem(c, c->X, c->X, 8, c->Dtmp));
+    CHK(trsm(c, c->M, c->X, 0));
+    CHK(elem(c, c->X, c->Y, 6, nullptr, c->ipvt));
+    CHK(gemm(c, c->UUL, 1, c->Y, 0, c->Gn));
+  } else {
+    CHK(gemm(c, c->UUL, 1, c->Q, 0, c->X));
+    CHK(elem(c, c->X, c->X, 9, c->Dtmp));
+    CHK(trsm(c, c->M, c->X, 1));
+    CHK(elem(c, c->X, c->Y, 7, nullptr, c->ipvt));
+    CHK(gemm(c, c->Y, 0, c->UUR, 1, c->Gn));
+  }
+  return Z2_OK;
+}
+
+static int drift_replace(z2_ctx *c, int accumulate) {
+  if (c->mixed)
+    hipLaunchKernelGGL((k_drift_replace<float>), dim3(c->nch), dim3(256), 0, c->stream, (const double *)c->Gn,
+                       c->Gf, c->xmax, c->xmean, c->N, accumulate);
+  else
+    hipLaunchKernelGGL((k_drift_replace<double>), dim3(c->nch), dim3(256), 0, c->stream, (const double *)c->Gn,
+                       c->G, c->xmax, c->xmean, c->N, accumulate);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+static inline size_t slot(const z2_ctx *c, int st) { return (size_t)st * c->nch * c->N * c->N; }
+static inline size_t slotd(const z2_ctx *c, int st) { return (size_t)st * c->nch * c->N; }
+
+extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
+  if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect preortho time slice %d", nt);
+  const int st = nt / c->w;
+  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
+  if (nt != 0) {
+    if (c->mixed) CHK(cvt(c, (const float *)c->UURf, c->UUR));
+    CHK(stab_UR(c));
+  }
+  CHK(dcopy(c, c->URl + slot(c, st), c->UUR, nn));
+  CHK(dcopy(c, c->VRl + slot(c, st), c->VUR, nn));
+  CHK(dcopy(c, c->DRl + slotd(c, st), c->DUR, nd));
+  if (nt == c->T) {
+    CHK(stab_green(c, nt));
+    if (c->mixed) CHK(cvt(c, (const double *)c->Gn, c->Gf));
+    else CHK(dcopy(c, c->G, c->Gn, nn));
+  }
+  if (c->mixed) CHK(cvt(c, (const double *)c->UUR, c->UURf));
+  return Z2_OK;
+}
+
+extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
+  if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect ortholeft time slice %d", nt);
+  const int st = nt / c->w;
+  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
+  CHK(dcopy(c, c->UUR, c->URl + slot(c, st), nn));
+  CHK(dcopy(c, c->VUR, c->VRl + slot(c, st), nn));
+  CHK(dcopy(c, c->DUR, c->DRl + slotd(c, st), nd));
+  if (nt != c->T) {
+    if (c->mixed) CHK(cvt(c, (const float *)c->UULf, c->UUL));
+    CHK(stab_UL(c));
+    CHK(stab_green(c, nt));
+    CHK(drift_replace(c, flag));
+  }
+  CHK(dcopy(c, c->ULl + slot(c, st), c->UUL, nn));
+  CHK(dcopy(c, c->VLl + slot(c, st), c->VUL, nn));
+  CHK(dcopy(c, c->DLl + slotd(c, st), c->DUL, nd));
+  if (c->mixed) CHK(cvt(c, (const double *)c->UUL, c->UULf));
+  return Z2_OK;
+}
+
+extern "C" int z2_wrap_R(z2_ctx *c, int32_t nt, int32_t flag) {
+  if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect orthoright time slice %d", nt);
+  const int st = nt / c->w;
+  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
+  CHK(dcopy(c, c->UUL, c->ULl + slot(c, st), nn));
+  CHK(dcopy(c, c->VUL, c->VLl + slot(c, st), nn));
+  CHK(dcopy(c, c->DUL, c->DLl + slotd(c, st), nd));
+  if (nt != 0) {
+    if (c->mixed) CHK(cvt(c, (const float *)c->UURf, c->UUR));
+    CHK(stab_UR(c));
+    CHK(stab_green(c, nt));
+    CHK(drift_replace(c, flag));
+  }
+  CHK(dcopy(c, c->URl + slot(c, st), c->UUR, nn));
+  CHK(dcopy(c, c->VRl + slot(c, st), c->VUR, nn));
+  CHK(dcopy(c, c->DRl + slotd(c, st), c->DUR, nd));
+  if (c->mixed) CHK(cvt(c, (const double *)c->UUR, c->UURf));
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- slices
+// propT_L + LocalK_prop_L (src/multiply.f90:79-93, src/localK.f90:347-373)
+template <typename R>
+static int slice_L_impl(z2_ctx *c, int nt) {
+  CHK(launch_flip<R>(c, nt, 1, 1));
+  KinArgs<R> a = kin_args<R>(c, nt);
+  a.m[0] = kmat<R>(PB<R>::G(c), +1, 1, FAC_EP, FAC_EM);  // G <- (flush((G*ep)*em)) B
+  a.m[1] = kmat<R>(PB<R>::UUL(c), +1, 0, FAC_EP);        // UUL <- (UUL*ep) B
+  CHK(launch_cols<R>(c, a, 2));
+  a.m[0] = kmat<R>(PB<R>::G(c), -1);                     // G <- B^-1 G
+  CHK(launch_rows<R>(c, a, 1));
+  return Z2_OK;
+}
+
+// LocalK_prop_R + propT_R (src/localK.f90:375-398, src/multiply.f90:95-108)
+template <typename R>
+static int slice_R_impl(z2_ctx *c, int nt) {
+  KinArgs<R> a = kin_args<R>(c, nt);
+  a.m[0] = kmat<R>(PB<R>::G(c), +1);                         // G <- B G
+  CHK(launch_rows<R>(c, a, 1));
+  a.m[0] = kmat<R>(PB<R>::G(c), -1);                         // G <- G B^-1
+  CHK(launch_cols<R>(c, a, 1));
+  CHK(launch_flip<R>(c, nt, 0, 0));
+  a.m[0] = kmat<R>(PB<R>::G(c), 0, 1, 0, 0, FAC_EP, FAC_EM);  // G <- (flush(G)*ep)*em
+  CHK(launch_elem_kin<R>(c, a, 1));
+  a.m[0] = kmat<R>(PB<R>::UUR(c), +1, 0, 0, 0, 0, 0, FAC_EP);  // UUR <- (B UUR)*ep
+  CHK(launch_rows<R>(c, a, 1));
+  return Z2_OK;
+}
+
+extern "C" int z2_slice_L(z2_ctx *c, int32_t nt) {
+  if (!c || nt < 1 || nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", nt);
+  return c->mixed ? slice_L_impl<float>(c, nt) : slice_L_impl<double>(c, nt);
+}
+
+extern "C" int z2_slice_R(z2_ctx *c, int32_t nt) {
+  if (!c || nt < 1 || nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", nt);
+  return c->mixed ? slice_R_impl<float>(c, nt) : slice_R_impl<double>(c, nt);
+}
+
+// propK_pre + propT_pre for all chains at slice nt (src/multiply.f90:8-15,65-77)
+static int pre_prop(z2_ctx *c, int nt) {
+  if (c->mixed) {
+    KinArgs<float> a = kin_args<float>(c, nt);
+    a.m[0] = kmat<float>(c->UURf, +1, 0, 0, 0, 0, 0, FAC_EP);
+    return launch_rows<float>(c, a, 1);
+  }
+  KinArgs<double> a = kin_args<double>(c, nt);
+  a.m[0] = kmat<double>(c->UUR, +1, 0, 0, 0, 0, 0, FAC_EP);
+  return launch_rows<double>(c, a, 1);
+}
+
+// ---------------------------------------------------------------- observables
+extern "C" int z2_obs_reset(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  HIPC(hipMemsetAsync(c->obs_acc, 0, sizeof(double) * c->nch * OBS_NACC, c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_obs_equal_calc(z2_ctx *c, int32_t nt) {
+  if (!c || nt < 1 || nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", nt);
+  CHK(prof_begin(c, Z2_K_OBS));
+  if (c->mixed) {
+    ObsArgs<float> o;
+    o.G = c->Gf; o.sig = c->sig; o.T = c->T; o.nt = nt; o.acc = c->obs_acc; o.cosT = c->d_cosT;
+    o.Lx = c->Lx; o.Ly = c->Ly; o.lat = c->lat;
+    hipLaunchKernelGGL((k_obs_equal<float>), dim3(c->nch), dim3(1024), sizeof(double) * c->N, c->stream, o);
+  } else {
+    ObsArgs<double> o;
+    o.G = c->G; o.sig = c->sig; o.T = c->T; o.nt = nt; o.acc = c->obs_acc; o.cosT = c->d_cosT;
+    o.Lx = c->Lx; o.Ly = c->Ly; o.lat = c->lat;
+    hipLaunchKernelGGL((k_obs_equal<double>), dim3(c->nch), dim3(1024), sizeof(double) * c->N, c->stream, o);
+  }
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_OBS));
+  return Z2_OK;
+}
+
+static int obs_ave(z2_ctx *c) {
+  hipLaunchKernelGGL(k_obs_ave, dim3((c->nch + 63) / 64), dim3(64), 0, c->stream, (const double *)c->obs_acc,
+                     c->obs_res, (const ChainConst *)c->dcc, c->N, c->nch);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- drivers
+// Local_sweep_reset's h ~ 0 synchronisation (src/local_sweep.f90:63-69)
+static int h_zero_sync(z2_ctx *c) {
+  for (int b = 0; b < c->nch; ++b) {
+    if (fabs(c->desc[b].h) > 1.0e-10) continue;
+    std::vector<double> s((size_t)c->T * c->N2);
+    CHK(z2_get_sigma(c, b, s.data()));
+    for (int nt = 1; nt < c->T; ++nt)
+      for (int ii = 0; ii < c->N2; ++ii) s[(size_t)nt * c->N2 + ii] = s[ii];
+    int32_t sd;
+    CHK(z2_get_seed(c, b, &sd));
+    CHK(z2_set_sigma(c, b, s.data(), sd));
+  }
+  return Z2_OK;
+}
+
+extern "C" int z2_pre(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (!c->have_fields) return set_err(Z2_ESTATE, "fields not initialised (z2_init_fields / z2_set_sigma)");
+  CHK(h_zero_sync(c));
+  CHK(z2_obs_reset(c));
+  CHK(z2_wrap_pre(c, 0));
+  for (int nt = 1; nt <= c->T; ++nt) {
+    CHK(pre_prop(c, nt));
+    if (nt % c->w == 0) CHK(z2_wrap_pre(c, nt));
+  }
+  c->have_pre = true;
+  return Z2_OK;
+}
+
+// One sweep as a list of steps (src/local_sweep.f90:106-150), so that two
+// contexts can be advanced in lockstep.
+enum StepKind { ST_WRAP_L, ST_WRAP_R, ST_OBS, ST_SLICE_L, ST_SLICE_R };
+struct Step {
+  StepKind kind;
+  int nt;
+};
+
+static void steps_L(const z2_ctx *c, std::vector<Step> &v) {
+  for (int nt = c->T; nt >= 1; --nt) {
+    if (nt % c->w == 0) v.push_back({ST_WRAP_L, nt});
+    v.push_back({ST_OBS, nt});
+    v.push_back({ST_SLICE_L, nt});
+  }
+  v.push_back({ST_WRAP_L, 0});
+}
+
+static void steps_R(const z2_ctx *c, std::vector<Step> &v) {
+  v.push_back({ST_WRAP_R, 0});
+  for (int nt = 1; nt <= c->T; ++nt) {
+    v.push_back({ST_SLICE_R, nt});
+    if (nt % c->w == 0) v.push_back({ST_WRAP_R, nt});
+    v.push_back({ST_OBS, nt});
+  }
+}
+
+static void sweep_steps(const z2_ctx *c, int is_beta, std::vector<Step> &v) {
+  for (int s = 0; s < c->p.Nsweep; ++s) {
+    if (is_beta) {
+      steps_L(c, v);
+      steps_R(c, v);
+    } else {
+      steps_R(c, v);
+      steps_L(c, v);
+    }
+  }
+}
+
+static int run_step(z2_ctx *c, const Step &st) {
+  switch (st.kind) {
+    case ST_WRAP_L: return z2_wrap_L(c, st.nt, 1);
+    case ST_WRAP_R: return z2_wrap_R(c, st.nt, 1);
+    case ST_OBS: return z2_obs_equal_calc(c, st.nt);
+    case ST_SLICE_L: return z2_slice_L(c, st.nt);
+    case ST_SLICE_R: return z2_slice_R(c, st.nt);
+  }
+  return set_err(Z2_EINVAL, "bad step");
+}
+
+// true when the step replaces G by a recomputed one (Wrap_L nt != Ltrot, Wrap_R nt != 0)
+static bool step_recomputes(const z2_ctx *c, const Step &st) {
+  return (st.kind == ST_WRAP_L && st.nt != c->T) || (st.kind == ST_WRAP_R && st.nt != 0);
+}
+
+extern "C" int z2_sweep(z2_ctx *c, int32_t is_beta) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_sweep");
+  CHK(h_zero_sync(c));
+  CHK(z2_obs_reset(c));
+  std::vector<Step> v;
+  sweep_steps(c, is_beta, v);
+  for (const Step &st : v) CHK(run_step(c, st));
+  CHK(obs_ave(c));
+  return Z2_OK;
+}
+
+template <typename Rb>
+static int maxdiff(z2_ctx *ref, z2_ctx *c, const Rb *Gc) {
+  hipLaunchKernelGGL((k_maxdiff<double, Rb>), dim3(c->nch), dim3(256), 0, c->stream, (const double *)ref->G,
+                     Gc, c->cmp, c->N);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *maxdiff_out) {
+  if (!c || !ref || !maxdiff_out) return set_err(Z2_EINVAL, "null argument");
+  if (ref->mixed || c->N != ref->N || c->T != ref->T || c->w != ref->w || c->nch != ref->nch ||
+      c->p.Nsweep != ref->p.Nsweep)
+    return set_err(Z2_EINVAL, "shadow needs an FP64 reference context of the same shape");
+  if (!c->have_pre || !ref->have_pre) return set_err(Z2_ESTATE, "z2_pre must run on both contexts");
+  CHK(h_zero_sync(c));
+  CHK(h_zero_sync(ref));
+  CHK(z2_obs_reset(c));
+  CHK(z2_obs_reset(ref));
+  HIPC(hipMemsetAsync(c->cmp, 0, sizeof(double) * c->nch, c->stream));
+  std::vector<Step> v;
+  sweep_steps(ref, is_beta, v);
+  c->force = ref->rec;
+  int rc = Z2_OK;
+  for (const Step &st : v) {
+    if (step_recomputes(ref, st)) {  // propagated G of both, before the recompute
+      HIPC(hipStreamSynchronize(ref->stream));
+      rc = c->mixed ? maxdiff<float>(ref, c, c->Gf) : maxdiff<double>(ref, c, c->G);
+      if (rc) break;
+    }
+    if ((rc = run_step(ref, st))) break;
+    HIPC(hipStreamSynchronize(ref->stream));
+    if ((rc = run_step(c, st))) break;
+    HIPC(hipStreamSynchronize(c->stream));
+  }
+  c->force = nullptr;
+  CHK(rc);
+  CHK(obs_ave(c));
+  CHK(obs_ave(ref));
+  HIPC(hipMemcpyAsync(maxdiff_out, c->cmp, sizeof(double) * c->nch, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- kernel test hooks
+// (include/z2dqmc_test.h) single-matrix runs of the dense kernels for parity tests.
+namespace {
+struct TmpBufs {
+  std::vector<void *> v;
+  ~TmpBufs() {
+    for (void *p : v) (void)hipFree(p);
+  }
+  template <typename T>
+  T *get(size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+    v.push_back(p);
+    return (T *)p;
+  }
+};
+}  // namespace
+
+extern "C" int z2_test_gemm(int32_t n, const double *A, int32_t ta, const double *B, int32_t tb,
+                            int32_t triA, int32_t triB, double *C) {
+  if (n < 1 || !A || !B || !C) return set_err(Z2_EINVAL, "bad argument");
+  TmpBufs t;
+  size_t nn = (size_t)n * n;
+  double *dA = t.get<double>(nn), *dB = t.get<double>(nn), *dC = t.get<double>(nn);
+  if (!dA || !dB || !dC) return set_err(Z2_ENOMEM, "alloc");
+  HIPC(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dB, B, nn * 8, hipMemcpyHostToDevice));
+  GemmArgs g;
+  g.A = dA; g.B = dB; g.C = dC; g.sA = g.sB = g.sC = (long)nn; g.N = n;
+  g.ta = ta; g.tb = tb; g.triA = triA; g.triB = triB; g.beta = 0.0;
+  dim3 grid((n + GT - 1) / GT, (n + GT - 1) / GT, 1);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, 0, g);
+  LAUNCH_CHECK();
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(C, dC, nn * 8, hipMemcpyDeviceToHost));
+  return Z2_OK;
+}
+
+extern "C" int z2_test_qrp(int32_t n, double *A, double *D, int32_t *ipvt, double *tau, double *Q) {
+  if (n < 1 || !A || !D || !ipvt || !tau) return set_err(Z2_EINVAL, "bad argument");
+  TmpBufs t;
+  size_t nn = (size_t)n * n;
+  double *dA = t.get<double>(nn), *dD = t.get<double>(n), *dt = t.get<double>(n), *dQ = t.get<double>(nn);
+  int *dp = t.get<int>(n);
+  if (!dA || !dD || !dt || !dp || !dQ) return set_err(Z2_ENOMEM, "alloc");
+  HIPC(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
+  QrpArgs q;
+  q.A = dA; q.tau = dt; q.ipvt = dp; q.D = dD; q.N = n;
+  size_t sh = sizeof(double) * (2 * n + 32) + sizeof(int) * 32;
+  hipLaunchKernelGGL(k_qrp, dim3(1), dim3(1024), sh, 0, q);
+  LAUNCH_CHECK();
+  if (Q) {
+    hipLaunchKernelGGL(k_orgqr, dim3(1), dim3(1024), 0, 0, (const double *)dA, (const double *)dt, dQ, n);
+    LAUNCH_CHECK();
+  }
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(A, dA, nn * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(D, dD, n * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(tau, dt, n * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(ipvt, dp, n * 4, hipMemcpyDeviceToHost));
+  if (Q) HIPC(hipMemcpy(Q, dQ, nn * 8, hipMemcpyDeviceToHost));
+  return Z2_OK;
+}
+
+extern "C" int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right) {
+  if (n < 1 || !R || !X) return set_err(Z2_EINVAL, "bad argument");
+  TmpBufs t;
+  size_t nn = (size_t)n * n;
+  double *dR = t.get<double>(nn), *dX = t.get<double>(nn);
+  if (!dR || !dX) return set_err(Z2_ENOMEM, "alloc");
+  HIPC(hipMemcpy(dR, R, nn * 8, hipMemcpyHostToDevice));
+  HIPC(hipMemcpy(dX, X, nn * 8, hipMemcpyHostToDevice));
+  dim3 grid((n + CBT - 1) / CBT, 1);
+  size_t sh = sizeof(double) * (CBT + 1) * n;
+  if (right) hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, 0, (const double *)dR, dX, n, n, 1, 1);
+  else hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, 0, (const double *)dR, dX, n, 1, n, 0);
+  LAUNCH_CHECK();
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(X, dX, nn * 8, hipMemcpyDeviceToHost));
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- kernel timing hooks
+extern "C" int z2_profile_kernel(z2_ctx *c, int32_t kind) {
+  if (!c || kind < 0 || kind > Z2_K_OBS) return set_err(Z2_EINVAL, "bad kernel class %d", kind);
+  HIPC(hipStreamSynchronize(c->stream));
+  c->prof_kind = kind;
+  c->prof_used = 0;
+  return Z2_OK;
+}
+
+extern "C" int z2_profile_read(z2_ctx *c, double *total_ms, int64_t *launches) {
+  if (!c || !total_ms || !launches) return set_err(Z2_EINVAL, "null argument");
+  HIPC(hipStreamSynchronize(c->stream));
+  double t = 0.0;
+  for (size_t k = 0; k < c->prof_used; ++k) {
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, c->prof_ev[k].first, c->prof_ev[k].second));
+    t += ms;
+  }
+  *total_ms = t;
+  *launches = (int64_t)c->prof_used;
+  return Z2_OK;
+}

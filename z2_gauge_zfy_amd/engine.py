@@ -1,0 +1,150 @@
+"""Python view of one engine context (a batch of chains sharing L, Ltrot, Nwrap).
+
+Thin plumbing over the C ABI; all work happens in the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import Z2ChainDesc, Z2Params, Z2_PREC_FP64, check, lib
+
+OBS_NAMES = ("flux_avg", "chi_flux", "pair_sc_q0", "cdw_pi", "cdw_shift", "cdw_ratio", "density")
+
+
+def _pd(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+@dataclass
+class ChainSpec:
+    """Per-chain couplings (reference paramC_sets.txt line 1 + Beta) and seed."""
+    RT: float = 1.0
+    J: float = 1.0
+    h: float = 0.5
+    mu: float = 0.0
+    Q: float = 1.0
+    Beta: float = 4.0
+    seed: int = 12345
+    absolute: int = 1
+
+
+class Engine:
+    def __init__(self, Nlx: int, Nly: int, Ltrot: int, Nwrap: int, chains, Nsweep: int = 1,
+                 precision: int = Z2_PREC_FP64):
+        self.params = Z2Params(Nlx, Nly, Ltrot, Nwrap, Nsweep, precision)
+        self.chains = list(chains)
+        arr = (Z2ChainDesc * len(self.chains))()
+        for k, s in enumerate(self.chains):
+            arr[k] = Z2ChainDesc(s.RT, s.J, s.h, s.mu, s.Q, s.Beta, int(s.seed), int(s.absolute))
+        self._ptr = C.c_void_p()
+        check(lib().z2_ctx_create(C.byref(self.params), len(self.chains), arr, C.byref(self._ptr)))
+        self.N = Nlx * Nly
+        self.T = Ltrot
+        self.nch = len(self.chains)
+
+    def close(self):
+        if getattr(self, "_ptr", None) is not None and self._ptr.value:
+            lib().z2_ctx_destroy(self._ptr)
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    # ---- drivers
+    def init_fields(self):
+        check(lib().z2_init_fields(self._ptr))
+
+    def set_sigma(self, chain: int, sigma: np.ndarray, seed: int):
+        s = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1)
+        assert s.size == 2 * self.N * self.T
+        check(lib().z2_set_sigma(self._ptr, chain, _pd(s), int(seed)))
+
+    def pre(self):
+        check(lib().z2_pre(self._ptr))
+
+    def sweep(self, is_beta: bool = True):
+        check(lib().z2_sweep(self._ptr, int(is_beta)))
+
+    def sweep_shadow(self, ref: "Engine", is_beta: bool = True) -> np.ndarray:
+        """One sweep in lockstep with an FP64 reference engine, replaying its
+        decisions; returns max|G_self - G_ref| per chain before each recompute."""
+        out = np.zeros(self.nch)
+        check(lib().z2_sweep_shadow(self._ptr, ref.ptr, int(is_beta), _pd(out)))
+        return out
+
+    def wrap_pre(self, nt):
+        check(lib().z2_wrap_pre(self._ptr, nt))
+
+    def wrap_L(self, nt, flag=1):
+        check(lib().z2_wrap_L(self._ptr, nt, flag))
+
+    def wrap_R(self, nt, flag=1):
+        check(lib().z2_wrap_R(self._ptr, nt, flag))
+
+    def slice_L(self, nt):
+        check(lib().z2_slice_L(self._ptr, nt))
+
+    def slice_R(self, nt):
+        check(lib().z2_slice_R(self._ptr, nt))
+
+    def sync(self):
+        check(lib().z2_sync(self._ptr))
+
+    # ---- state
+    def sigma(self, chain: int) -> np.ndarray:
+        s = np.zeros(2 * self.N * self.T)
+        check(lib().z2_get_sigma(self._ptr, chain, _pd(s)))
+        return s.reshape(self.T, 2 * self.N)
+
+    def seed(self, chain: int) -> int:
+        v = C.c_int32()
+        check(lib().z2_get_seed(self._ptr, chain, C.byref(v)))
+        return v.value
+
+    def green(self, chain: int) -> np.ndarray:
+        G = np.zeros((self.N, self.N), order="F")
+        check(lib().z2_get_green(self._ptr, chain, _pd(G)))
+        return G
+
+    def set_green(self, chain: int, G: np.ndarray):
+        A = np.asfortranarray(G, dtype=np.float64)
+        check(lib().z2_set_green(self._ptr, chain, _pd(A)))
+
+    def drift(self, chain: int):
+        a, b = C.c_double(), C.c_double()
+        check(lib().z2_get_drift(self._ptr, chain, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def obs_equal(self, chain: int) -> dict:
+        o = np.zeros(7)
+        check(lib().z2_get_obs_equal(self._ptr, chain, _pd(o)))
+        return dict(zip(OBS_NAMES, o.tolist()))
+
+    def counters(self, chain: int):
+        o = (C.c_uint64 * 2)()
+        check(lib().z2_get_counters(self._ptr, chain, o))
+        return int(o[0]), int(o[1])
+
+    def profile_kernel(self, name: str):
+        from ._lib import KERNEL_IDS
+        check(lib().z2_profile_kernel(self._ptr, KERNEL_IDS[name] if name else 0))
+
+    def profile_read(self):
+        t, n = C.c_double(), C.c_int64()
+        check(lib().z2_profile_read(self._ptr, C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(lib().z2_get_stream(self._ptr, C.byref(s)))
+        return s.value or 0
