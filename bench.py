@@ -89,15 +89,73 @@ def cpu_baseline(cfg, nslices):
                       f"max over processes {max(times):.2f} s"}
 
 
+def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay):
+    """Time `steps` sweeps of all chains after `warmup` sweeps; returns a dict."""
+    import z2_gauge_zfy_amd as z
+    L, T, beta, w, nch, hs, mu = CONFIGS[cfg]
+    N = L * L
+    eng = z.Engine(L, L, T, w, chain_specs(cfg), precision=precision)
+    if delay:
+        eng.set_delay(delay)
+    eng.init_fields()
+    eng.pre()
+    for _ in range(warmup):
+        eng.sweep(True)
+    eng.sync()
+    acc0 = sum(eng.counters(k)[0] for k in range(nch))
+    prop0 = sum(eng.counters(k)[1] for k in range(nch))
+    eng.profile_kernel(roofline_kernel)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.sweep(True)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    kms, klaunch = eng.profile_read()
+    eng.profile_kernel("")
+    acc1 = sum(eng.counters(k)[0] for k in range(nch))
+    prop1 = sum(eng.counters(k)[1] for k in range(nch))
+    eng.close()
+    return dict(dt=dt, sweeps_per_s=nch * steps / dt, flips_per_s=(prop1 - prop0) / dt,
+                acc_per_s=(acc1 - acc0) / dt, acceptance=(acc1 - acc0) / max(1, prop1 - prop0),
+                kms=kms, klaunch=klaunch)
+
+
+def shadow_max_dG(cfg, sweeps, warmup, delay):
+    """max |G_mixed - G_fp64| over chains and over the propagated G just before
+    every recompute, FP32 run replaying the FP64 run's decisions."""
+    import numpy as np
+    import z2_gauge_zfy_amd as z
+    L, T, beta, w, nch, hs, mu = CONFIGS[cfg]
+    ref = z.Engine(L, L, T, w, chain_specs(cfg), precision=z.Z2_PREC_FP64)
+    mix = z.Engine(L, L, T, w, chain_specs(cfg), precision=z.Z2_PREC_MIXED)
+    for e in (ref, mix):
+        if delay:
+            e.set_delay(delay)
+        e.init_fields()
+        e.pre()
+    d = np.zeros(nch)
+    for _ in range(warmup + sweeps):
+        d = np.maximum(d, mix.sweep_shadow(ref, True))
+    obs = {}
+    for name in ("density", "pair_sc_q0", "cdw_pi", "flux_avg", "chi_flux"):
+        obs[name] = max(abs(mix.obs_equal(k)[name] - ref.obs_equal(k)[name]) for k in range(nch))
+    ref.close()
+    mix.close()
+    return float(d.max()), obs
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
+    ap.add_argument("--precision", default="both", choices=("both", "mixed", "fp64"))
     ap.add_argument("--roofline-kernel", default="qrp")
+    ap.add_argument("--delay", type=int, default=0, help="delay depth k (0: engine default)")
     ap.add_argument("--cpu-slices", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-shadow", action="store_true")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -109,29 +167,16 @@ def main():
     import z2_gauge_zfy_amd as z
     L, T, beta, w, nch, hs, mu = CONFIGS[a.config]
     N = L * L
-    specs = chain_specs(a.config)
-    eng = z.Engine(L, L, T, w, specs)
-    eng.init_fields()
-    eng.pre()
-    for _ in range(a.warmup):
-        eng.sweep(True)
-    eng.sync()
-    acc0 = sum(eng.counters(k)[0] for k in range(nch))
-    prop0 = sum(eng.counters(k)[1] for k in range(nch))
-    eng.profile_kernel(a.roofline_kernel)
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        eng.sweep(True)
-    eng.sync()
-    dt = time.perf_counter() - t0
-    kms, klaunch = eng.profile_read()
-    eng.profile_kernel("")
-    acc1 = sum(eng.counters(k)[0] for k in range(nch))
-    prop1 = sum(eng.counters(k)[1] for k in range(nch))
-    sweeps_per_s = nch * a.steps / dt
+    res = {}
+    if a.precision in ("both", "fp64"):
+        res["fp64"] = run_mode(a.config, z.Z2_PREC_FP64, a.steps, a.warmup, a.roofline_kernel, a.delay)
+    if a.precision in ("both", "mixed"):
+        res["mixed"] = run_mode(a.config, z.Z2_PREC_MIXED, a.steps, a.warmup, a.roofline_kernel, a.delay)
+    head = res["mixed"] if "mixed" in res else res["fp64"]
+    head_name = "mixed" if "mixed" in res else "fp64"
 
     bound, work = kernel_work(a.roofline_kernel, N)
-    avg_s = kms / 1e3 / max(1, klaunch)
+    avg_s = head["kms"] / 1e3 / max(1, head["klaunch"])
     per_launch = work * nch
     if bound == "hbm":
         achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
@@ -139,29 +184,37 @@ def main():
         achieved, peak, unit = per_launch / avg_s / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
     out = {
         "metric": METRIC,
-        "value": sweeps_per_s,
+        "value": head["sweeps_per_s"],
         "unit": "sweeps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": dt / a.steps * 1e3,
+        "ms_per_step": head["dt"] / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head_name == "mixed" else "f64",
         "data": "synthetic (seeded random initial links, absolute=1)",
         "config": {"workload": f"{a.config}: {nch} chains, L={L} (N={N}), beta={beta}, Ltrot={T}, "
-                               f"Nwrap={w}, h in {list(hs)}, J=1, t=1, mu={mu}, FP64",
+                               f"Nwrap={w}, h in {list(hs)}, J=1, t=1, mu={mu}",
                    "n_chains": nch, "L": L, "Ltrot": T, "beta": beta, "Nwrap": w,
-                   "precision": "fp64"},
-        "flip_updates_per_s": (prop1 - prop0) / dt,
-        "accepted_updates_per_s": (acc1 - acc0) / dt,
-        "roofline": {"kernel": a.roofline_kernel, "bound": bound, "achieved": achieved, "peak": peak,
-                     "unit": unit, "frac": achieved / peak, "traffic": None,
-                     "avg_launch_us": avg_s * 1e6, "launches": klaunch,
-                     "kernel_share_of_step": (kms / 1e3) / dt},
+                   "precision": head_name},
+        "flip_updates_per_s": head["flips_per_s"],
+        "accepted_updates_per_s": head["acc_per_s"],
+        "acceptance": head["acceptance"],
     }
-    eng.close()
+    for k, r in res.items():
+        out[f"{k}_sweeps_per_s"] = r["sweeps_per_s"]
+    if "mixed" in res and "fp64" in res:
+        out["mixed_over_fp64"] = res["mixed"]["sweeps_per_s"] / res["fp64"]["sweeps_per_s"]
+    out["roofline"] = {"kernel": a.roofline_kernel, "bound": bound, "achieved": achieved, "peak": peak,
+                       "unit": unit, "frac": achieved / peak, "traffic": None,
+                       "avg_launch_us": avg_s * 1e6, "launches": head["klaunch"],
+                       "kernel_share_of_step": (head["kms"] / 1e3) / head["dt"]}
+    if a.precision == "both" and not a.no_shadow:
+        dg, dobs = shadow_max_dG(a.config, 1, a.warmup, a.delay)
+        out["max_dG"] = dg
+        out["max_dobs"] = dobs
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.config, a.cpu_slices)
     print(json.dumps(out), flush=True)

@@ -1,0 +1,294 @@
+! Fortran API surface of the reference (module and procedure names of
+! src/model.f90, src/stabilization.f90, src/obser_equal.f90, src/local_sweep.f90)
+! over the gfx950 engine.  Propagator and WrapList live on the GPU; the Fortran
+! types keep the reference's argument lists and carry only the chain index.
+module DQMC_Model_mod
+    use iso_c_binding
+    use z2dqmc_c
+    implicit none
+    ! paramC_sets.txt (src/calc_basic.f90:12-44)
+    real(kind=8) :: RT, J, h, mu, Q, Beta, Dtau
+    integer :: Nlx, Nly, Ltrot, NlxTherm, NlyTherm, LtrotTherm
+    integer :: Nwrap, Nbin, Nsweep, Nthermal, Nwarm, Nglobal, absolute, Lq, Ndim, Nst
+    logical :: is_tau, is_warm, is_global
+    integer :: precision_mode = 0
+    ! all chains of this process (one reference MPI rank each)
+    integer :: n_chains = 1
+    type(c_ptr) :: ctx = c_null_ptr
+contains
+    ! read_input + Params_set (src/calc_basic.f90:50-102)
+    subroutine read_input()
+        open(unit=20, file='paramC_sets.txt', status='old')
+        read(20, *) RT, J, h, mu, Q
+        read(20, *) Nlx, Nly, Ltrot, Beta
+        read(20, *) NlxTherm, NlyTherm, LtrotTherm
+        read(20, *) Nwrap, Nbin, Nsweep
+        read(20, *) is_tau, Nthermal
+        read(20, *) is_warm, Nwarm
+        read(20, *) is_global, Nglobal
+        read(20, *) absolute
+        close(20)
+        Lq = Nlx * Nly
+        Ndim = Lq
+        Dtau = Beta / dble(Ltrot)
+        if (mod(Ltrot, Nwrap) /= 0) then
+            write(6, *) "Ltrot must be a multiple of Nwrap"; stop 1
+        endif
+        Nst = Ltrot / Nwrap
+    end subroutine read_input
+
+    ! Model_init (src/model.f90:16-36): one chain per line of seeds.txt
+    subroutine Model_init(iseed)
+        integer, intent(out) :: iseed
+        type(z2_params) :: p
+        type(z2_chain_desc), allocatable :: d(:)
+        integer, allocatable :: seeds(:)
+        integer :: k, ios, s, conf0
+        call read_input()
+        if (is_global .or. is_warm .or. is_tau) then
+            write(6, *) "note: Wolff global updates, boson warm-up and time-displaced measurements", &
+                        " are not part of this engine build; running local sweeps only"
+        endif
+        if (NlxTherm /= Nlx .or. NlyTherm /= Nly .or. LtrotTherm /= Ltrot) then
+            write(6, *) "thermalisation-size transfer is not supported by this driver"; stop 1
+        endif
+        open(unit=30, file='confin.txt', status='old')
+        read(30, *) conf0
+        close(30)
+        if (conf0 /= 0) then
+            write(6, *) "restart from confin.txt is not supported by this driver"; stop 1
+        endif
+        allocate(seeds(4096))
+        n_chains = 0
+        open(unit=10, file='seeds.txt', status='old')
+        do
+            read(10, *, iostat=ios) s
+            if (ios /= 0) exit
+            n_chains = n_chains + 1
+            seeds(n_chains) = s
+            if (n_chains == size(seeds)) exit
+        enddo
+        close(10)
+        if (n_chains == 0) then
+            write(6, *) "seeds.txt holds no seed"; stop 1
+        endif
+        allocate(d(n_chains))
+        do k = 1, n_chains
+            d(k) = z2_chain_desc(RT, J, h, mu, Q, Beta, int(seeds(k), c_int32_t), int(absolute, c_int32_t))
+        enddo
+        p = z2_params(int(Nlx, c_int32_t), int(Nly, c_int32_t), int(Ltrot, c_int32_t), &
+                      int(Nwrap, c_int32_t), int(Nsweep, c_int32_t), int(precision_mode, c_int32_t))
+        call z2_check(z2_ctx_create(p, int(n_chains, c_int32_t), d, ctx), "z2_ctx_create")
+        call z2_check(z2_init_fields(ctx), "z2_init_fields")
+        call z2_check(z2_get_seed(ctx, 0_c_int32_t, iseed), "z2_get_seed")
+    end subroutine Model_init
+
+    ! Model_clear -> conf_out (src/model.f90:38-45, src/fields.f90:509-557)
+    subroutine Model_clear(iseed)
+        integer, intent(in) :: iseed
+        real(c_double), allocatable :: sig(:)
+        integer(c_int32_t) :: s
+        integer :: k, i
+        allocate(sig(2 * Lq * Ltrot))
+        open(unit=35, file='confout.txt', status='unknown')
+        do k = 0, n_chains - 1
+            call z2_check(z2_get_seed(ctx, int(k, c_int32_t), s), "z2_get_seed")
+            call z2_check(z2_get_sigma(ctx, int(k, c_int32_t), sig), "z2_get_sigma")
+            write(35, *) s
+            do i = 1, size(sig)
+                write(35, *) sig(i)
+            enddo
+        enddo
+        close(35)
+        call z2_check(z2_ctx_destroy(ctx), "z2_ctx_destroy")
+        ctx = c_null_ptr
+        if (iseed == 0) continue
+    end subroutine Model_clear
+end module DQMC_Model_mod
+
+module ProcessMatrix
+    implicit none
+    ! device-resident: the Fortran objects only name the chain they stand for
+    type :: Propagator
+        integer :: chain = 0
+        real(kind=8) :: Xmaxm = 0.d0, Xmeanm = 0.d0
+    contains
+        procedure :: make => Prop_make
+    end type Propagator
+    type :: WrapList
+        integer :: chain = 0
+    contains
+        procedure :: make => Wrlist_make
+    end type WrapList
+contains
+    subroutine Prop_make(this)
+        class(Propagator), intent(inout) :: this
+        this%Xmaxm = 0.d0
+        this%Xmeanm = 0.d0
+    end subroutine Prop_make
+    subroutine Wrlist_make(this)
+        class(WrapList), intent(inout) :: this
+        this%chain = 0
+    end subroutine Wrlist_make
+end module ProcessMatrix
+
+module Stabilize_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod, only: ctx
+    use ProcessMatrix
+    implicit none
+contains
+    subroutine Stabilize_init()
+    end subroutine Stabilize_init
+    subroutine Stabilize_clear()
+    end subroutine Stabilize_clear
+    ! src/stabilization.f90:374-476 (all chains at once)
+    subroutine Wrap_pre(Prop, WrList, nt)
+        class(Propagator), intent(inout) :: Prop
+        class(WrapList), intent(inout) :: WrList
+        integer, intent(in) :: nt
+        call z2_check(z2_wrap_pre(ctx, int(nt, c_int32_t)), "Wrap_pre")
+    end subroutine Wrap_pre
+    subroutine Wrap_L(Prop, WrList, nt, flag)
+        class(Propagator), intent(inout) :: Prop
+        class(WrapList), intent(inout) :: WrList
+        integer, intent(in) :: nt
+        character(len=*), optional, intent(in) :: flag
+        integer(c_int32_t) :: f
+        f = 0
+        if (present(flag)) f = 1
+        call z2_check(z2_wrap_L(ctx, int(nt, c_int32_t), f), "Wrap_L")
+    end subroutine Wrap_L
+    subroutine Wrap_R(Prop, WrList, nt, flag)
+        class(Propagator), intent(inout) :: Prop
+        class(WrapList), intent(inout) :: WrList
+        integer, intent(in) :: nt
+        character(len=*), optional, intent(in) :: flag
+        integer(c_int32_t) :: f
+        f = 0
+        if (present(flag)) f = 1
+        call z2_check(z2_wrap_R(ctx, int(nt, c_int32_t), f), "Wrap_R")
+    end subroutine Wrap_R
+end module Stabilize_mod
+
+module ObserEqual_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod, only: ctx, n_chains
+    implicit none
+    ! src/obser_equal.f90:7-20 (chain-averaged, as after the per-bin MPI_REDUCE)
+    type, public :: ObserEqual
+        real(kind=8) :: flux_avg, chi_flux, pair_sc_q0, cdw_pi, cdw_shift, cdw_ratio, density
+    contains
+        procedure :: make => Obs_equal_make
+        procedure :: reset => Obs_equal_reset
+        procedure :: ave => Obs_equal_ave
+    end type ObserEqual
+contains
+    subroutine Obs_equal_make(this)
+        class(ObserEqual), intent(inout) :: this
+        call this%reset()
+    end subroutine Obs_equal_make
+    subroutine Obs_equal_reset(this)
+        class(ObserEqual), intent(inout) :: this
+        this%flux_avg = 0.d0; this%chi_flux = 0.d0; this%pair_sc_q0 = 0.d0
+        this%cdw_pi = 0.d0; this%cdw_shift = 0.d0; this%cdw_ratio = 0.d0; this%density = 0.d0
+    end subroutine Obs_equal_reset
+    ! Obs_equal%ave (src/obser_equal.f90:50-71) + the per-bin rank average of
+    ! m_process_obs_equal (src/fourier_trans.f90:280-315)
+    subroutine Obs_equal_ave(this, Nobs)
+        class(ObserEqual), intent(inout) :: this
+        integer, intent(in) :: Nobs
+        real(c_double) :: o(7)
+        integer :: k
+        call this%reset()
+        do k = 0, n_chains - 1
+            call z2_check(z2_get_obs_equal(ctx, int(k, c_int32_t), o), "z2_get_obs_equal")
+            this%flux_avg = this%flux_avg + o(1)
+            this%chi_flux = this%chi_flux + o(2)
+            this%pair_sc_q0 = this%pair_sc_q0 + o(3)
+            this%cdw_pi = this%cdw_pi + o(4)
+            this%cdw_shift = this%cdw_shift + o(5)
+            this%cdw_ratio = this%cdw_ratio + o(6)
+            this%density = this%density + o(7)
+        enddo
+        this%flux_avg = this%flux_avg / dble(n_chains)
+        this%chi_flux = this%chi_flux / dble(n_chains)
+        this%pair_sc_q0 = this%pair_sc_q0 / dble(n_chains)
+        this%cdw_pi = this%cdw_pi / dble(n_chains)
+        this%cdw_shift = this%cdw_shift / dble(n_chains)
+        this%cdw_ratio = this%cdw_ratio / dble(n_chains)
+        this%density = this%density / dble(n_chains)
+        if (Nobs < 0) continue
+    end subroutine Obs_equal_ave
+end module ObserEqual_mod
+
+module LocalSweep_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod
+    use ProcessMatrix
+    use ObserEqual_mod
+    implicit none
+    type :: LocalSweep
+    contains
+        procedure :: init => Local_sweep_init
+        procedure :: clear => Local_sweep_clear
+        procedure :: pre => Local_sweep_pre
+        procedure :: sweep => Local_sweep
+        procedure, nopass :: ctrl_print_l => Local_control_print
+    end type LocalSweep
+    type(ObserEqual), allocatable :: Obs_equal
+contains
+    subroutine Local_sweep_init(this)
+        class(LocalSweep), intent(inout) :: this
+        allocate(Obs_equal)
+        call Obs_equal%make()
+    end subroutine Local_sweep_init
+    subroutine Local_sweep_clear(this)
+        class(LocalSweep), intent(inout) :: this
+        deallocate(Obs_equal)
+    end subroutine Local_sweep_clear
+    ! src/local_sweep.f90:87-104
+    subroutine Local_sweep_pre(this, PropU, PropD, WrU, WrD)
+        class(LocalSweep), intent(inout) :: this
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(WrapList), intent(inout) :: WrU, WrD
+        call z2_check(z2_pre(ctx), "z2_pre")
+    end subroutine Local_sweep_pre
+    ! src/local_sweep.f90:152-195 (all chains; iseed returns chain 0's state)
+    subroutine Local_sweep(this, PropU, PropD, WrU, WrD, iseed, is_beta, toggle)
+        class(LocalSweep), intent(inout) :: this
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(WrapList), intent(inout) :: WrU, WrD
+        integer, intent(inout) :: iseed
+        logical, intent(inout) :: is_beta
+        logical, intent(in) :: toggle
+        integer(c_int32_t) :: b, s
+        real(c_double) :: xmax, xmean
+        b = 0
+        if (is_beta) b = 1
+        call z2_check(z2_sweep(ctx, b), "z2_sweep")
+        call Obs_equal%ave(2 * Ltrot * Nsweep)
+        call z2_check(z2_get_seed(ctx, 0_c_int32_t, s), "z2_get_seed")
+        iseed = s
+        call z2_check(z2_get_drift(ctx, 0_c_int32_t, xmax, xmean), "z2_get_drift")
+        PropU%Xmaxm = xmax; PropU%Xmeanm = xmean
+        PropD%Xmaxm = xmax; PropD%Xmeanm = xmean
+        if (toggle) continue
+    end subroutine Local_sweep
+    ! acceptance over all chains (src/local_sweep.f90:197-213)
+    subroutine Local_control_print(toggle)
+        logical, intent(in) :: toggle
+        integer(c_int64_t) :: cnt(2), acc, prop
+        integer :: k
+        acc = 0; prop = 0
+        do k = 0, n_chains - 1
+            call z2_check(z2_get_counters(ctx, int(k, c_int32_t), cnt), "z2_get_counters")
+            acc = acc + cnt(1); prop = prop + cnt(2)
+        enddo
+        write(50, *) 'Accept_Klocal_shift                            :', dble(acc) / dble(max(prop, 1_c_int64_t))
+        if (toggle) continue
+    end subroutine Local_control_print
+end module LocalSweep_mod

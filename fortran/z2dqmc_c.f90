@@ -1,0 +1,170 @@
+! iso_c_binding interfaces of the gfx950 engine (include/z2dqmc.h).
+! Every procedure returns the C status (0 = ok); z2_error_message() returns
+! z2_last_error() as a Fortran string.
+module z2dqmc_c
+    use iso_c_binding
+    implicit none
+
+    integer(c_int32_t), parameter :: Z2_PREC_FP64 = 0, Z2_PREC_MIXED = 1
+
+    type, bind(C) :: z2_params
+        integer(c_int32_t) :: Nlx, Nly, Ltrot, Nwrap, Nsweep, precision
+    end type z2_params
+
+    type, bind(C) :: z2_chain_desc
+        real(c_double) :: RT, J, h, mu, Q, Beta
+        integer(c_int32_t) :: seed, absolute
+    end type z2_chain_desc
+
+    interface
+        function z2_last_error() bind(C, name="z2_last_error") result(p)
+            import :: c_ptr
+            type(c_ptr) :: p
+        end function
+        function z2_ctx_create(p, n, chains, ctx) bind(C, name="z2_ctx_create") result(rc)
+            import :: c_int, c_int32_t, c_ptr, z2_params, z2_chain_desc
+            type(z2_params), intent(in) :: p
+            integer(c_int32_t), value :: n
+            type(z2_chain_desc), intent(in) :: chains(*)
+            type(c_ptr), intent(out) :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_ctx_destroy(ctx) bind(C, name="z2_ctx_destroy") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_init_fields(ctx) bind(C, name="z2_init_fields") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_set_sigma(ctx, chain, sigma, seed) bind(C, name="z2_set_sigma") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain, seed
+            real(c_double), intent(in) :: sigma(*)
+            integer(c_int) :: rc
+        end function
+        function z2_get_sigma(ctx, chain, sigma) bind(C, name="z2_get_sigma") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: sigma(*)
+            integer(c_int) :: rc
+        end function
+        function z2_get_seed(ctx, chain, seed) bind(C, name="z2_get_seed") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            integer(c_int32_t), intent(out) :: seed
+            integer(c_int) :: rc
+        end function
+        function z2_pre(ctx) bind(C, name="z2_pre") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_sweep(ctx, is_beta) bind(C, name="z2_sweep") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: is_beta
+            integer(c_int) :: rc
+        end function
+        function z2_wrap_pre(ctx, nt) bind(C, name="z2_wrap_pre") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_wrap_L(ctx, nt, flag) bind(C, name="z2_wrap_L") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt, flag
+            integer(c_int) :: rc
+        end function
+        function z2_wrap_R(ctx, nt, flag) bind(C, name="z2_wrap_R") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt, flag
+            integer(c_int) :: rc
+        end function
+        function z2_slice_L(ctx, nt) bind(C, name="z2_slice_L") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_slice_R(ctx, nt) bind(C, name="z2_slice_R") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_obs_equal_calc(ctx, nt) bind(C, name="z2_obs_equal_calc") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_get_obs_equal(ctx, chain, out) bind(C, name="z2_get_obs_equal") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: out(7)
+            integer(c_int) :: rc
+        end function
+        function z2_get_drift(ctx, chain, xmax, xmean) bind(C, name="z2_get_drift") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: xmax, xmean
+            integer(c_int) :: rc
+        end function
+        function z2_get_counters(ctx, chain, out) bind(C, name="z2_get_counters") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            integer(c_int64_t), intent(out) :: out(2)
+            integer(c_int) :: rc
+        end function
+        function z2_get_green(ctx, chain, G) bind(C, name="z2_get_green") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: G(*)
+            integer(c_int) :: rc
+        end function
+        function c_strlen(s) bind(C, name="strlen") result(n)
+            import :: c_ptr, c_size_t
+            type(c_ptr), value :: s
+            integer(c_size_t) :: n
+        end function
+    end interface
+
+contains
+
+    function z2_error_message() result(msg)
+        character(len=:), allocatable :: msg
+        type(c_ptr) :: p
+        character(kind=c_char), pointer :: chars(:)
+        integer :: n, i
+        p = z2_last_error()
+        n = int(c_strlen(p))
+        call c_f_pointer(p, chars, [n])
+        allocate(character(len=n) :: msg)
+        do i = 1, n
+            msg(i:i) = chars(i)
+        enddo
+    end function z2_error_message
+
+    subroutine z2_check(rc, what)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: what
+        if (rc /= 0) then
+            write(6, *) "z2dqmc error in ", what, ": ", z2_error_message()
+            stop 1
+        endif
+    end subroutine z2_check
+
+end module z2dqmc_c

@@ -101,6 +101,11 @@ int z2_get_drift(z2_ctx *ctx, int32_t chain, double *xmax, double *xmean);
 int z2_get_obs_equal(z2_ctx *ctx, int32_t chain, double out[7]);
 /* cumulative local-flip counts: accepted, proposed */
 int z2_get_counters(z2_ctx *ctx, int32_t chain, uint64_t out[2]);
+/* delay depth k of the local-update kernel: accepted rank-2 updates are kept
+ * pending (in LDS) and applied to G in blocks of k (default min(16, kmax(N)),
+ * kmax set by the LDS size; FP64 results do not depend on k, the updates are
+ * applied in acceptance order) */
+int z2_set_delay(z2_ctx *ctx, int32_t k);
 /* HIP stream of the context (hipStream_t as void*) for timing by the caller */
 int z2_get_stream(z2_ctx *ctx, void **stream);
 

@@ -36,7 +36,7 @@ def test_gemm(L, n, ta, tb, triA, triB):
     np.testing.assert_allclose(Cm, a @ b, rtol=1e-12, atol=1e-12 * n)
 
 
-@pytest.mark.parametrize("n", [16, 100])
+@pytest.mark.parametrize("n", [16, 100, 256, 600])
 def test_qrp_matches_oracle(L, n):
     from z2_gauge_zfy_amd._lib import check
     rng = np.random.default_rng(n)

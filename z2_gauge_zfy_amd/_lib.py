@@ -67,6 +67,7 @@ _SIGS = {
     "z2_get_obs_equal": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "z2_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "z2_set_delay": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     # kernel test hooks (include/z2dqmc_test.h)

@@ -9,14 +9,15 @@
 // and folded into the cached 2x2 blocks of every bond, and evaluation resumes
 // after it.  Proposals before the first accepted one were evaluated against the
 // exact state the serial loop would have seen, so the accept/reject history is
-// the serial one.  The N x N Green's function itself is not touched here: the
-// recorded updates are applied, in order, by the next kinetic kernel (flush).
+// the serial one.  The recorded updates are applied to G in acceptance order:
+// by this kernel whenever `kdelay` of them are pending (the delay depth), and
+// the remaining ones by the next kinetic kernel (flush prologue).
 #pragma once
 #include "z2_common.hpp"
 
 template <typename R>
 struct FlipArgs {
-  const R *G;             // chain stride N*N
+  R *G;                   // chain stride N*N
   int8_t *sig;            // chain stride T*2N
   int32_t *seed;          // per chain
   R *Tbuf, *Vbuf;         // per chain stride upd_stride
@@ -28,6 +29,8 @@ struct FlipArgs {
   int T, nt;
   int descending;         // 1: bonds 2N..1 (left pass), 0: 1..2N (right pass)
   int mu_on_read;         // 1: G entries are read as (g*ep)*em (left pass, propT_L before flips)
+  int kdelay;             // delay depth: pending updates before an in-kernel flush
+  int *mu_done;           // out: 1 when an in-kernel flush applied the mu factors to G
   const uint32_t *force;  // optional: decisions to replay, bit k = proposal k, chain stride dwords
   uint32_t *record;       // optional: decisions taken, same layout
   int dwords;
@@ -45,11 +48,11 @@ __device__ __forceinline__ R flip_read(const R *G, int i, int j, int N, int mu, 
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
+__global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   const int N = a.lat.N, N2 = a.lat.N2;
   const int chain = blockIdx.x;
   const ChainConst c = a.cc[chain];
-  const R *G = a.G + (size_t)chain * N * N;
+  R *G = a.G + (size_t)chain * N * N;
   int8_t *sigc = a.sig + (size_t)chain * a.T * N2;
   int8_t *sig_t = sigc + (size_t)(a.nt - 1) * N2;
   const int tp = (a.nt - 2 + a.T) % a.T, tn = a.nt % a.T;  // 0-based neighbour slices
@@ -62,16 +65,16 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
   double *o12 = dg + N;                               // 2N  : G_p1p2 per bond
   double *o21 = o12 + N2;                             // 2N  : G_p2p1 per bond
   double *rnd = o21 + N2;                             // 2N  : random number per proposal index
-  double *TA = rnd + N2;                              // 2N  : T of the current accept [c*N + i]
-  double *VA = TA + N2;                               // 2N  : V of the current accept [c*N + i]
-  int *sg = reinterpret_cast<int *>(VA + N2);         // 2N  : current slice field
+  double *PT = rnd + N2;                              // kdelay x 4N : pending T0,T1,V0,V1
+  int *sg = reinterpret_cast<int *>(PT + (size_t)4 * N * a.kdelay);  // 2N : current slice field
   __shared__ int s_kmin;
   __shared__ double s_pi[4], s_d[4];
   __shared__ int s_p[2];
-  __shared__ int s_nacc;
+  __shared__ int s_nacc, s_tot;
   __shared__ uint32_t s_dec[64];  // decision bits of this slice (2N <= 2048)
 
-  const int mu = a.mu_on_read;
+  int mu = a.mu_on_read;
+  int mu_done = 0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) dg[i] = (double)flip_read<R>(G, i, i, N, mu, c);
   const uint32_t seed0 = (uint32_t)a.seed[chain] & 0x7fffffffu;
   for (int b = threadIdx.x; b < N2; b += blockDim.x) {
@@ -83,7 +86,10 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
     uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[b + 1];
     rnd[b] = (double)(uint32_t)(x & 0x7fffffffull) / 2147483648.0;
   }
-  if (threadIdx.x == 0) s_nacc = 0;
+  if (threadIdx.x == 0) {
+    s_nacc = 0;
+    s_tot = 0;
+  }
   for (int q = threadIdx.x; q < 64; q += blockDim.x) s_dec[q] = 0u;
   const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
   __syncthreads();
@@ -145,25 +151,26 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
       s_p[1] = p2;
       sg[b] = -s_old;
       s_dec[kmin >> 5] |= 1u << (kmin & 31);
+      s_tot = s_tot + 1;
     }
     __syncthreads();
+    double *TA = PT + (size_t)4 * N * na;  // slot of this accept: T0, T1, V0, V1
     {
       const int p1 = s_p[0], p2 = s_p[1];
       const double Pi11 = s_pi[0], Pi21 = s_pi[1], Pi12 = s_pi[2], Pi22 = s_pi[3];
       const double D11 = s_d[0], D21 = s_d[1], D12 = s_d[2], D22 = s_d[3];
       for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        // current G(i,p1), G(i,p2), G(p1,i), G(p2,i): earlier accepts applied in order
+        // current G(i,p1), G(i,p2), G(p1,i), G(p2,i): pending accepts applied in order
         double c1 = (double)flip_read<R>(G, i, p1, N, mu, c);
         double c2 = (double)flip_read<R>(G, i, p2, N, mu, c);
         double r1 = (double)flip_read<R>(G, p1, i, N, mu, c);
         double r2 = (double)flip_read<R>(G, p2, i, N, mu, c);
         for (int k = 0; k < na; ++k) {
-          double ti0 = (double)Tb[(2 * k) * N + i], ti1 = (double)Tb[(2 * k + 1) * N + i];
-          double tp10 = (double)Tb[(2 * k) * N + p1], tp11 = (double)Tb[(2 * k + 1) * N + p1];
-          double tp20 = (double)Tb[(2 * k) * N + p2], tp21 = (double)Tb[(2 * k + 1) * N + p2];
-          double vi0 = (double)Vb[(2 * k) * N + i], vi1 = (double)Vb[(2 * k + 1) * N + i];
-          double vp10 = (double)Vb[(2 * k) * N + p1], vp11 = (double)Vb[(2 * k + 1) * N + p1];
-          double vp20 = (double)Vb[(2 * k) * N + p2], vp21 = (double)Vb[(2 * k + 1) * N + p2];
+          const double *P = PT + (size_t)4 * N * k;
+          double ti0 = P[i], ti1 = P[N + i];
+          double tp10 = P[p1], tp11 = P[N + p1], tp20 = P[p2], tp21 = P[N + p2];
+          double vi0 = P[2 * N + i], vi1 = P[3 * N + i];
+          double vp10 = P[2 * N + p1], vp11 = P[3 * N + p1], vp20 = P[2 * N + p2], vp21 = P[3 * N + p2];
           c1 = c1 - (ti0 * vp10 + ti1 * vp11);
           c2 = c2 - (ti0 * vp20 + ti1 * vp21);
           r1 = r1 - (tp10 * vi0 + tp11 * vi1);
@@ -183,8 +190,8 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
         }
         TA[i] = t0;
         TA[N + i] = t1;
-        VA[i] = v0;
-        VA[N + i] = v1;
+        TA[2 * N + i] = v0;
+        TA[3 * N + i] = v1;
         Tb[(2 * na) * N + i] = (R)t0;
         Tb[(2 * na + 1) * N + i] = (R)t1;
         Vb[(2 * na) * N + i] = (R)v0;
@@ -193,6 +200,7 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
     }
     __syncthreads();
     // fold the update into the cached 2x2 blocks
+    const double *VA = TA + 2 * N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
       double d = TA[i] * VA[i] + TA[N + i] * VA[N + i];
       dg[i] = dg[i] - d;
@@ -206,6 +214,33 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
     }
     if (threadIdx.x == 0) s_nacc = na + 1;
     __syncthreads();
+    if (na + 1 >= a.kdelay) {
+      // flush the pending updates into G in acceptance order (and mu, once)
+      const int np = na + 1;
+      for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+        const int col = idx / N, r = idx - col * N;
+        R x = G[idx];
+        if (mu) {
+          x = x * (R)c.ep;
+          x = x * (R)c.em;
+        }
+        for (int k = 0; k < np; ++k) {
+          const double *P = PT + (size_t)4 * N * k;
+          R t0 = (R)P[r], t1 = (R)P[N + r];
+          R v0 = (R)P[2 * N + col], v1 = (R)P[3 * N + col];
+          R pp = t0 * v0;
+          R qq = t1 * v1;
+          R dd = pp + qq;
+          x = x - dd;
+        }
+        G[idx] = x;
+      }
+      if (mu) mu_done = 1;
+      mu = 0;
+      __syncthreads();
+      if (threadIdx.x == 0) s_nacc = 0;
+      __syncthreads();
+    }
     start = kmin + 1;
   }
   __syncthreads();
@@ -214,9 +249,10 @@ __global__ void __launch_bounds__(256) k_flip(FlipArgs<R> a) {
     for (int q = threadIdx.x; q < a.dwords; q += blockDim.x) a.record[(size_t)chain * a.dwords + q] = s_dec[q];
   if (threadIdx.x == 0) {
     a.nacc[chain] = s_nacc;
+    a.mu_done[chain] = mu_done;
     uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[N2];
     a.seed[chain] = (int32_t)(uint32_t)(x & 0x7fffffffull);
-    a.acc_count[2 * chain] += (unsigned long long)s_nacc;
+    a.acc_count[2 * chain] += (unsigned long long)s_tot;
     a.acc_count[2 * chain + 1] += (unsigned long long)N2;
   }
 }

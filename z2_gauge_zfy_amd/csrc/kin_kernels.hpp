@@ -18,6 +18,7 @@ struct KinMat {
   int nflag;      // +1 / -1 rotation, 0: none
   int flush;      // apply delayed updates
   int pre1, pre2; // factors before the flush
+  int pre_cond;   // skip pre1/pre2 for chains whose flip kernel already applied them
   int mid1, mid2; // factors after the flush
   int post;       // factor after the rotation
 };
@@ -31,6 +32,7 @@ struct KinArgs {
   const R *Tbuf;         // delayed updates: chain stride upd_stride, [(k*2+c)*N + row]
   const R *Vbuf;         //                                          [(k*2+c)*N + col]
   const int *nacc;       // accepted updates per chain
+  const int *mu_done;    // per chain: mu factors already applied by the flip kernel
   long upd_stride;
   LatticeDev lat;
 };
@@ -46,8 +48,10 @@ __device__ __forceinline__ R apply_fac(R x, int sel, const ChainConst &c) {
 template <typename R>
 __device__ __forceinline__ R kin_prologue(R x, int r, int col, const KinMat<R> &km,
                                           const KinArgs<R> &a, const ChainConst &c, int chain) {
-  x = apply_fac<R>(x, km.pre1, c);
-  x = apply_fac<R>(x, km.pre2, c);
+  if (!(km.pre_cond && a.mu_done[chain])) {
+    x = apply_fac<R>(x, km.pre1, c);
+    x = apply_fac<R>(x, km.pre2, c);
+  }
   if (km.flush) {
     const int N = a.lat.N;
     const int na = a.nacc[chain];

@@ -174,6 +174,241 @@ __global__ void __launch_bounds__(1024) k_qrp(QrpArgs a) {
   }
 }
 
+// Register-blocked variant of k_qrp / k_orgqr for N <= 64*RPL: every lane
+// keeps its RPL rows of the Householder vector in registers and a wavefront
+// updates CPW columns at once (one load/store round trip per column group).
+#define QR_CPW 8
+#define QR_REG_THREADS 512  // 2 waves per SIMD: room for CPW*RPL doubles per lane
+
+// apply H = I - tau v v^T (v in registers vr[t] for rows r = i + lane + 64 t)
+// to columns j of M[i:, j]; optional ZLAQP2 norm downdating.
+template <int RPL>
+__device__ __forceinline__ void apply_h_cols(double *M, int N, int i, const double (&vr)[RPL], double tau,
+                                             int jbeg, int jstep, double *vn1, double *vn2, int lane) {
+  const double tol3z = 1.4901161193847656e-08;  // sqrt(2^-53)
+  for (int j0 = jbeg; j0 < N; j0 += QR_CPW * jstep) {
+    double x[QR_CPW][RPL];
+    double d[QR_CPW];
+#pragma unroll
+    for (int cc = 0; cc < QR_CPW; ++cc) {
+      const int j = j0 + cc * jstep;
+      d[cc] = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = i + lane + 64 * t;
+        x[cc][t] = (j < N && r < N) ? M[r + (size_t)j * N] : 0.0;
+      }
+    }
+    if (tau != 0.0) {
+#pragma unroll
+      for (int cc = 0; cc < QR_CPW; ++cc)
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) d[cc] += x[cc][t] * vr[t];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int cc = 0; cc < QR_CPW; ++cc) d[cc] += __shfl_xor(d[cc], off, 64);
+#pragma unroll
+      for (int cc = 0; cc < QR_CPW; ++cc) {
+        const int j = j0 + cc * jstep;
+        const double tt = -tau * d[cc];
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int r = i + lane + 64 * t;
+          x[cc][t] = x[cc][t] + vr[t] * tt;
+          if (j < N && r < N) M[r + (size_t)j * N] = x[cc][t];
+        }
+      }
+    }
+    if (vn1) {
+#pragma unroll
+      for (int cc = 0; cc < QR_CPW; ++cc) {
+        const int j = j0 + cc * jstep;
+        if (j >= N) continue;
+        const double aij = __shfl(x[cc][0], 0, 64);  // row i sits in lane 0, t = 0
+        const double v1 = vn1[j];
+        if (v1 != 0.0) {
+          double rr = fabs(aij) / v1;
+          double temp = 1.0 - rr * rr;
+          temp = temp < 0.0 ? 0.0 : temp;
+          double qq = v1 / vn2[j];
+          double temp2 = temp * qq * qq;
+          if (temp2 <= tol3z) {
+            double s2 = 0.0;
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) {
+              const int r = i + lane + 64 * t;
+              if (r > i && r < N) s2 += x[cc][t] * x[cc][t];
+            }
+            s2 = wave_sum(s2);
+            if (lane == 0) {
+              vn1[j] = (i < N - 1) ? sqrt(s2) : 0.0;
+              vn2[j] = vn1[j];
+            }
+          } else if (lane == 0) {
+            vn1[j] = v1 * sqrt(temp);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(QR_REG_THREADS) k_qrp_reg(QrpArgs a) {
+  const int N = a.N;
+  const int chain = blockIdx.x;
+  double *A = a.A + (size_t)chain * N * N;
+  double *tau = a.tau + (size_t)chain * N;
+  int *ipvt = a.ipvt + (size_t)chain * N;
+  extern __shared__ unsigned char smem_raw[];
+  double *vn1 = reinterpret_cast<double *>(smem_raw);
+  double *vn2 = vn1 + N;
+  double *redv = vn2 + N;
+  int *redi = reinterpret_cast<int *>(redv + 32);
+  __shared__ double s_tau, s_beta, s_scale;
+  __shared__ int s_pvt;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int j = wid; j < N; j += nw) {
+    const double *col = A + (size_t)j * N;
+    double s = 0.0;
+    for (int r = lane; r < N; r += 64) s += col[r] * col[r];
+    s = wave_sum(s);
+    if (lane == 0) {
+      vn1[j] = sqrt(s);
+      vn2[j] = vn1[j];
+      ipvt[j] = j + 1;
+    }
+  }
+  __syncthreads();
+  for (int i = 0; i < N; ++i) {
+    double best = -1.0;
+    int bi = N;
+    for (int j = i + threadIdx.x; j < N; j += blockDim.x) {
+      double v = vn1[j];
+      if (v > best) { best = v; bi = j; }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      double ov = __shfl_xor(best, off, 64);
+      int oi = __shfl_xor(bi, off, 64);
+      if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if (lane == 0) { redv[wid] = best; redi[wid] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double bv = redv[0];
+      int bj = redi[0];
+      for (int w = 1; w < nw; ++w)
+        if (redv[w] > bv || (redv[w] == bv && redi[w] < bj)) { bv = redv[w]; bj = redi[w]; }
+      s_pvt = bj;
+    }
+    __syncthreads();
+    const int pvt = s_pvt;
+    double *ci = A + (size_t)i * N;
+    if (pvt != i) {
+      double *cp = A + (size_t)pvt * N;
+      for (int r = threadIdx.x; r < N; r += blockDim.x) {
+        double t = cp[r];
+        cp[r] = ci[r];
+        ci[r] = t;
+      }
+      if (threadIdx.x == 0) {
+        int it = ipvt[pvt];
+        ipvt[pvt] = ipvt[i];
+        ipvt[i] = it;
+        vn1[pvt] = vn1[i];
+        vn2[pvt] = vn2[i];
+      }
+      __syncthreads();
+    }
+    // reflector: wave 0 computes the norm of A[i+1:, i] (dlarfg)
+    if (wid == 0) {
+      double s = 0.0;
+      for (int r = i + 1 + lane; r < N; r += 64) s += ci[r] * ci[r];
+      s = wave_sum(s);
+      if (lane == 0) {
+        double xnorm = sqrt(s);
+        double alpha = ci[i];
+        if (xnorm == 0.0 || i == N - 1) {
+          s_tau = 0.0;
+          s_beta = alpha;
+          s_scale = 1.0;
+        } else {
+          double xa = fabs(alpha), w_ = fmax(xa, xnorm), z_ = fmin(xa, xnorm);
+          double q = z_ / w_;
+          double nrm = w_ * sqrt(1.0 + q * q);
+          double beta = -copysign(nrm, alpha);
+          s_tau = (beta - alpha) / beta;
+          s_scale = 1.0 / (alpha - beta);
+          s_beta = beta;
+        }
+      }
+    }
+    __syncthreads();
+    const double tau_i = s_tau, sc = s_scale;
+    double vr[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = i + lane + 64 * t;
+      vr[t] = (r == i) ? 1.0 : (r < N ? ci[r] * sc : 0.0);
+    }
+    __syncthreads();  // every wave holds v before wave 0 overwrites column i below
+    apply_h_cols<RPL>(A, N, i, vr, tau_i, i + 1 + wid, nw, vn1, vn2, lane);
+    // the reflector itself (wave 0 owns the write-back of column i)
+    if (wid == 0) {
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = i + lane + 64 * t;
+        if (r > i && r < N) ci[r] = vr[t];
+      }
+      if (lane == 0) {
+        ci[i] = s_beta;
+        tau[i] = tau_i;
+      }
+    }
+    __syncthreads();
+  }
+  for (int r = threadIdx.x; r < N; r += blockDim.x) a.D[(size_t)chain * N + r] = fabs(A[r + (size_t)r * N]);
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+    int j = idx / N, r = idx - j * N;
+    if (r <= j) A[idx] = A[idx] / a.D[(size_t)chain * N + r];
+  }
+}
+
+template <int RPL>
+__global__ void __launch_bounds__(QR_REG_THREADS) k_orgqr_reg(const double *Ain, const double *tauin, double *Qout,
+                                                    int N) {
+  const int chain = blockIdx.x;
+  const double *A = Ain + (size_t)chain * N * N;
+  const double *tau = tauin + (size_t)chain * N;
+  double *Q = Qout + (size_t)chain * N * N;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+    int j = idx / N, r = idx - j * N;
+    Q[idx] = (r == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int i = N - 1; i >= 0; --i) {
+    const double ti = tau[i];
+    const double *v = A + (size_t)i * N;
+    double vr[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = i + lane + 64 * t;
+      vr[t] = (r == i) ? 1.0 : (r < N ? v[r] : 0.0);
+    }
+    apply_h_cols<RPL>(Q, N, i, vr, ti, i + 1 + wid, nw, nullptr, nullptr, lane);
+    double *qi = Q + (size_t)i * N;
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+      if (r < i) qi[r] = 0.0;
+      else if (r == i) qi[r] = 1.0 - ti;
+      else qi[r] = -ti * v[r];
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ ORGQR
 // Q (N x N) from the reflectors (below the diagonal of A) and tau; writes Q to
 // Q (may not alias A).  ZUNG2R order: H(N-1) applied first to the unit matrix.

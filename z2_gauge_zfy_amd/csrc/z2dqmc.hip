@@ -74,7 +74,10 @@ struct z2_ctx {
   int32_t *seed = nullptr;
   double *Tbuf = nullptr, *Vbuf = nullptr;
   long upd_stride = 0;
-  int *nacc = nullptr;
+  int *nacc = nullptr, *mu_done = nullptr;
+  int kdelay = 16;  // delay depth of the flip kernel
+  int kdelay_max = 16;
+  size_t lds_max = 65536;
   unsigned long long *accc = nullptr;
   double *xmax = nullptr, *xmean = nullptr;
   double *obs_acc = nullptr, *obs_res = nullptr;
@@ -319,6 +322,7 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
       if ((rc = dalloc(c, &c->Vbuf, B * c->upd_stride))) break;
     }
     if ((rc = dalloc(c, &c->nacc, B))) break;
+    if ((rc = dalloc(c, &c->mu_done, B))) break;
     if ((rc = dalloc(c, &c->accc, 2 * B))) break;
     if ((rc = dalloc(c, &c->xmax, B))) break;
     if ((rc = dalloc(c, &c->xmean, B))) break;
@@ -333,6 +337,18 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if ((rc = dalloc(c, &c->Dtmp, B * N))) break;
     if ((rc = dalloc(c, &c->ipvt, B * N))) break;
     c->dwords = (N2 + 31) / 32;
+    {
+      int dev = 0, lds = 0;
+      if (hipGetDevice(&dev) == hipSuccess &&
+          hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
+        c->lds_max = (size_t)lds;
+      // flip kernel LDS: (N + 6N) doubles + 2N ints + 4N doubles per pending update
+      size_t base = sizeof(double) * (size_t)(N + 3 * N2) + sizeof(int) * N2 + 1024;
+      long cap = c->lds_max > base ? (long)((c->lds_max - base) / (sizeof(double) * 4 * (size_t)N)) : 0;
+      if (cap < 1) { rc = set_err(Z2_EINVAL, "N too large for the flip kernel's LDS"); break; }
+      c->kdelay_max = (int)(cap < N2 ? cap : N2);
+      c->kdelay = c->kdelay_max < 16 ? c->kdelay_max : 16;
+    }
     if ((rc = dalloc(c, &c->rec, B * c->dwords))) break;
     if ((rc = dalloc(c, &c->cmp, B))) break;
     c->mixed = p->precision == Z2_PREC_MIXED;
@@ -357,7 +373,8 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
         hipMemset(c->DRl, 0, sizeof(double) * L * B * N) || hipMemset(c->DLl, 0, sizeof(double) * L * B * N) ||
         hipMemset(c->accc, 0, sizeof(unsigned long long) * 2 * B) || hipMemset(c->xmax, 0, sizeof(double) * B) ||
         hipMemset(c->xmean, 0, sizeof(double) * B) || hipMemset(c->obs_acc, 0, sizeof(double) * B * OBS_NACC) ||
-        hipMemset(c->obs_res, 0, sizeof(double) * B * RES_N) || hipMemset(c->nacc, 0, sizeof(int) * B)) {
+        hipMemset(c->obs_res, 0, sizeof(double) * B * RES_N) || hipMemset(c->nacc, 0, sizeof(int) * B) ||
+        hipMemset(c->mu_done, 0, sizeof(int) * B)) {
       rc = set_err(Z2_EHIP, "memset");
       break;
     }
@@ -391,6 +408,13 @@ extern "C" int z2_ctx_info(const z2_ctx *c, int32_t *N, int32_t *nch, int32_t *p
   if (N) *N = c->N;
   if (nch) *nch = c->nch;
   if (prec) *prec = c->p.precision;
+  return Z2_OK;
+}
+
+extern "C" int z2_set_delay(z2_ctx *c, int32_t k) {
+  if (!c || k < 1 || k > c->kdelay_max)
+    return set_err(Z2_EINVAL, "delay depth must be in [1, %d] for N = %d", c ? c->kdelay_max : 0, c ? c->N : 0);
+  c->kdelay = k;
   return Z2_OK;
 }
 
@@ -583,6 +607,7 @@ static KinArgs<R> kin_args(z2_ctx *c, int nt) {
   a.Tbuf = PB<R>::Tb(c);
   a.Vbuf = PB<R>::Vb(c);
   a.nacc = c->nacc;
+  a.mu_done = c->mu_done;
   a.upd_stride = c->upd_stride;
   a.lat = c->lat;
   return a;
@@ -600,10 +625,12 @@ static KinMat<R> kmat(R *m, int nflag, int flush = 0, int pre1 = 0, int pre2 = 0
   k.mid1 = mid1;
   k.mid2 = mid2;
   k.post = post;
+  k.pre_cond = 0;
   return k;
 }
 
 static const int KCB = 16;  // columns per block, row-rotation kernel
+static constexpr int FLIP_THREADS = 512;  // must not exceed k_flip's __launch_bounds__
 static const int KRB = 16;  // rows per block, column-rotation kernel
 
 template <typename R>
@@ -655,13 +682,17 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
   f.nt = nt;
   f.descending = descending;
   f.mu_on_read = mu_on_read;
+  f.kdelay = c->kdelay;
+  f.mu_done = c->mu_done;
   f.force = c->force;
   f.record = c->rec;
   f.dwords = c->dwords;
   f.lat = c->lat;
-  size_t sh = sizeof(double) * (c->N + 5 * c->N2) + sizeof(int) * c->N2;
+  size_t sh = sizeof(double) * ((size_t)c->N + 3 * c->N2 + (size_t)4 * c->N * c->kdelay) + sizeof(int) * c->N2;
+  static_assert(FLIP_THREADS <= 512, "k_flip is compiled for at most 512 threads per block");
+  if (sh > 65536) HIPC(hipFuncSetAttribute((const void *)k_flip<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   CHK(prof_begin(c, Z2_K_FLIP));
-  hipLaunchKernelGGL((k_flip<R>), dim3(c->nch), dim3(256), sh, c->stream, f);
+  hipLaunchKernelGGL((k_flip<R>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
   LAUNCH_CHECK();
   CHK(prof_end(c, Z2_K_FLIP));
   return Z2_OK;
@@ -745,6 +776,31 @@ static int gemm(z2_ctx *c, const double *A, int ta, const double *B, int tb, dou
   return Z2_OK;
 }
 
+// register-blocked QR kernels for N <= 512, the plain ones above
+static int launch_qrp(int N, dim3 grid, size_t sh, hipStream_t st, QrpArgs q) {
+  const int rpl = (N + 63) / 64;
+  switch (rpl) {
+#define QCASE(R) case R: hipLaunchKernelGGL((k_qrp_reg<R>), grid, dim3(QR_REG_THREADS), sh, st, q); break;
+    QCASE(1) QCASE(2) QCASE(3) QCASE(4) QCASE(5) QCASE(6) QCASE(7) QCASE(8)
+#undef QCASE
+    default: hipLaunchKernelGGL(k_qrp, grid, dim3(1024), sh, st, q);
+  }
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+static int launch_orgqr(int N, dim3 grid, hipStream_t st, const double *A, const double *tau, double *Q) {
+  const int rpl = (N + 63) / 64;
+  switch (rpl) {
+#define OCASE(R) case R: hipLaunchKernelGGL((k_orgqr_reg<R>), grid, dim3(QR_REG_THREADS), 0, st, A, tau, Q, N); break;
+    OCASE(1) OCASE(2) OCASE(3) OCASE(4) OCASE(5) OCASE(6) OCASE(7) OCASE(8)
+#undef OCASE
+    default: hipLaunchKernelGGL(k_orgqr, grid, dim3(1024), 0, st, A, tau, Q, N);
+  }
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
 static int qrp(z2_ctx *c, double *A, double *D) {
   QrpArgs q;
   q.A = A;
@@ -754,16 +810,14 @@ static int qrp(z2_ctx *c, double *A, double *D) {
   q.N = c->N;
   size_t sh = sizeof(double) * (2 * c->N + 32) + sizeof(int) * 32;
   CHK(prof_begin(c, Z2_K_QRP));
-  hipLaunchKernelGGL(k_qrp, dim3(c->nch), dim3(1024), sh, c->stream, q);
-  LAUNCH_CHECK();
+  CHK(launch_qrp(c->N, dim3(c->nch), sh, c->stream, q));
   CHK(prof_end(c, Z2_K_QRP));
   return Z2_OK;
 }
 
 static int orgqr(z2_ctx *c, const double *A, double *Qo) {
   CHK(prof_begin(c, Z2_K_ORGQR));
-  hipLaunchKernelGGL(k_orgqr, dim3(c->nch), dim3(1024), 0, c->stream, A, (const double *)c->tau, Qo, c->N);
-  LAUNCH_CHECK();
+  CHK(launch_orgqr(c->N, dim3(c->nch), c->stream, A, (const double *)c->tau, Qo));
   CHK(prof_end(c, Z2_K_ORGQR));
   return Z2_OK;
 }
@@ -917,6 +971,7 @@ static int slice_L_impl(z2_ctx *c, int nt) {
   CHK(launch_flip<R>(c, nt, 1, 1));
   KinArgs<R> a = kin_args<R>(c, nt);
   a.m[0] = kmat<R>(PB<R>::G(c), +1, 1, FAC_EP, FAC_EM);  // G <- (flush((G*ep)*em)) B
+  a.m[0].pre_cond = 1;                                    // mu already in G if the flip kernel flushed
   a.m[1] = kmat<R>(PB<R>::UUL(c), +1, 0, FAC_EP);        // UUL <- (UUL*ep) B
   CHK(launch_cols<R>(c, a, 2));
   a.m[0] = kmat<R>(PB<R>::G(c), -1);                     // G <- B^-1 G
@@ -1183,12 +1238,8 @@ extern "C" int z2_test_qrp(int32_t n, double *A, double *D, int32_t *ipvt, doubl
   QrpArgs q;
   q.A = dA; q.tau = dt; q.ipvt = dp; q.D = dD; q.N = n;
   size_t sh = sizeof(double) * (2 * n + 32) + sizeof(int) * 32;
-  hipLaunchKernelGGL(k_qrp, dim3(1), dim3(1024), sh, 0, q);
-  LAUNCH_CHECK();
-  if (Q) {
-    hipLaunchKernelGGL(k_orgqr, dim3(1), dim3(1024), 0, 0, (const double *)dA, (const double *)dt, dQ, n);
-    LAUNCH_CHECK();
-  }
+  CHK(launch_qrp(n, dim3(1), sh, 0, q));
+  if (Q) CHK(launch_orgqr(n, dim3(1), 0, (const double *)dA, (const double *)dt, dQ));
   HIPC(hipDeviceSynchronize());
   HIPC(hipMemcpy(A, dA, nn * 8, hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(D, dD, n * 8, hipMemcpyDeviceToHost));

@@ -135,6 +135,9 @@ class Engine:
         check(lib().z2_get_counters(self._ptr, chain, o))
         return int(o[0]), int(o[1])
 
+    def set_delay(self, k: int):
+        check(lib().z2_set_delay(self._ptr, int(k)))
+
     def profile_kernel(self, name: str):
         from ._lib import KERNEL_IDS
         check(lib().z2_profile_kernel(self._ptr, KERNEL_IDS[name] if name else 0))
