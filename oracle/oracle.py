@@ -92,6 +92,9 @@ def lib():
             "z2o_sweep": (None, [V, C.c_int]),
             "z2o_global_sweep": (None, [V]),
             "z2o_sweep_L_partial": (None, [V, C.c_int]),
+            "z2o_pass_L": (None, [V]),
+            "z2o_pass_R": (None, [V, C.c_int]),
+            "z2o_dyn_measure": (None, [V]),
             "z2o_slice_L": (None, [V, C.c_int]),
             "z2o_slice_R": (None, [V, C.c_int]),
             "z2o_get_seed": (I32, [V]),
@@ -262,6 +265,15 @@ class Chain:
 
     def sweep(self, toggle=False):
         lib().z2o_sweep(self.ptr, int(toggle))
+
+    def pass_L(self):
+        lib().z2o_pass_L(self.ptr)
+
+    def pass_R(self, toggle=False):
+        lib().z2o_pass_R(self.ptr, int(toggle))
+
+    def dyn_measure(self):
+        lib().z2o_dyn_measure(self.ptr)
 
     def sweep_L_partial(self, nslices: int):
         lib().z2o_sweep_L_partial(self.ptr, nslices)

@@ -1840,6 +1840,23 @@ static void sweep_R(z2o_chain *c, int toggle, int *Nobs, int *Nobst) {
   }
 }
 
+/* single passes and the time-displaced measurement on their own (test hooks) */
+void z2o_pass_L(z2o_chain *c) {
+  int Nobs = 0;
+  sweep_L(c, &Nobs);
+}
+
+void z2o_pass_R(z2o_chain *c, int toggle) {
+  int Nobs = 0, Nobst = 0;
+  sweep_R(c, toggle, &Nobs, &Nobst);
+}
+
+void z2o_dyn_measure(z2o_chain *c) {
+  if (!c->have_dyn) return;
+  memset(c->otau, 0, sizeof(double) * 7 * (size_t)c->N * c->T);
+  dyn_sweep(c);
+}
+
 void z2o_sweep(z2o_chain *c, int toggle) {
   int Nobs = 0, Nobst = 0;
   sweep_reset(c, toggle);

@@ -95,6 +95,11 @@ void z2o_global_sweep(z2o_chain *c);          /* src/global_update.f90:313-337 *
 /* the first nslices steps of a left pass (src/local_sweep.f90:111-120); a timing
  * sample for the CPU baseline */
 void z2o_sweep_L_partial(z2o_chain *c, int nslices);
+/* the left pass (src/local_sweep.f90:106-124), the right pass (:126-150), and
+ * Dyn%reset + Dyn%sweep_R alone on the current state (src/dynamics.f90:41-62) */
+void z2o_pass_L(z2o_chain *c);
+void z2o_pass_R(z2o_chain *c, int toggle);
+void z2o_dyn_measure(z2o_chain *c);
 /* single-slice pieces for fine-grained parity (src/localK.f90:347-398) */
 void z2o_slice_L(z2o_chain *c, int nt);
 void z2o_slice_R(z2o_chain *c, int nt);

@@ -272,3 +272,59 @@ def test_half_filling_density_at_stabilization():
     np.testing.assert_allclose(np.diag(c.green()), 0.5, atol=1e-12)
     c.sweep()
     np.testing.assert_allclose(np.diag(c.green()), 0.5, atol=1e-12)
+
+
+# ------------------------------------------------------ time-displaced (a17-a18)
+def test_time_displaced_measurement_dense():
+    """Dyn%sweep_R: G(tau,0) = B(tau,0) G(0), G(0,tau) = -(I-G(0)) B(tau,0)^{-1},
+    G(tau,tau) = B(tau,0) G(0) B(tau,0)^{-1} with recomputes every Nwrap slices;
+    the seven tau observables against dense formulas (small beta)."""
+    L, T, beta, w, mu, RT = 4, 8, 0.8, 4, 0.2, 1.0
+    p = O.make_params(RT=RT, J=1.0, h=0.5, mu=mu, Nlx=L, Nly=L, Ltrot=T, Beta=beta, Nwrap=w,
+                      is_tau=1)
+    m = O.Model(p)
+    c = O.Chain(m, 2468)
+    c.pre()
+    c.pass_L()
+    sigma = c.sigma()
+    G0 = c.green()
+    c.dyn_measure()
+    got = c.obs_tau()
+    N = L * L
+    ep, em = m.mu_factors()
+    Bs = [ep * _dense_B(m, sigma, t, 0.5 * (beta / T) * RT) for t in range(1, T + 1)]
+    Bbeta = np.eye(N)
+    for B in Bs:
+        Bbeta = B @ Bbeta
+    np.testing.assert_allclose(G0, np.linalg.inv(np.eye(N) + Bbeta), rtol=1e-10, atol=1e-12)
+    t = m.tables()
+    imj, Lb = t["imj"], t["L_bonds"]
+    want = {k: np.zeros((N, T)) for k in ("spin", "pair_s", "pair_d", "charge_s", "charge_d",
+                                           "single", "curxx")}
+    Bt0 = np.eye(N)
+    I = np.eye(N)
+    for nt in range(1, T + 1):
+        Gt0 = Bt0 @ G0
+        G0t = -(I - G0) @ np.linalg.inv(Bt0)
+        Gtt = Bt0 @ G0 @ np.linalg.inv(Bt0)
+        tt = 2 * (1 - np.diag(Gtt))
+        t0 = 2 * (1 - np.diag(G0))
+        for j in range(N):
+            for i in range(N):
+                q = imj[i, j] - 1
+                want["spin"][q, nt - 1] += sigma[nt - 1, i] * sigma[0, j]
+                want["pair_d"][q, nt - 1] += 8 * G0t[j, i] ** 2
+                want["pair_s"][q, nt - 1] += 8 * G0t[j, i] ** 2
+                a = G0t[j, i] * Gt0[i, j]
+                want["charge_d"][q, nt - 1] += -4 * a
+                want["charge_s"][q, nt - 1] += -4 * a + 4 * tt[i] * t0[j]
+                want["single"][q, nt - 1] += 4 * Gt0[i, j]
+                ip, jp = Lb[i, 1] - 1, Lb[j, 1] - 1
+                term = (-G0t[j, ip] * Gt0[i, jp] - G0t[jp, i] * Gt0[ip, j] + G0t[j, i] * Gt0[ip, jp]
+                        + G0t[jp, ip] * Gt0[i, j])
+                want["curxx"][q, nt - 1] += -4 * RT * RT * term
+        Bt0 = Bs[nt - 1] @ Bt0
+    for k in want:
+        np.testing.assert_allclose(got[k], want[k], rtol=1e-8, atol=1e-10, err_msg=k)
+    xmax, _ = c.dyn_drift()
+    assert np.all(xmax < 1e-9)  # no flips during the measurement: recomputes agree

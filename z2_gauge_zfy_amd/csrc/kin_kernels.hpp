@@ -20,12 +20,12 @@ struct KinMat {
   int pre1, pre2; // factors before the flush
   int pre_cond;   // skip pre1/pre2 for chains whose flip kernel already applied them
   int mid1, mid2; // factors after the flush
-  int post;       // factor after the rotation
+  int post, post2; // factors after the rotation
 };
 
 template <typename R>
 struct KinArgs {
-  KinMat<R> m[2];        // grid.z selects the matrix
+  KinMat<R> m[4];        // grid.z selects the matrix
   const int8_t *sig;     // sigma base: chain stride T*2N, slice (nt-1)*2N
   int T, nt;
   const ChainConst *cc;  // per chain
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
   }
   for (int idx = threadIdx.x; idx < ncol * N; idx += blockDim.x) {
     int cc = idx / N, r = idx - cc * N;
-    M[r + (size_t)(col0 + cc) * N] = apply_fac<R>(tile[cc * N + r], km.post, c);
+    M[r + (size_t)(col0 + cc) * N] = apply_fac<R>(apply_fac<R>(tile[cc * N + r], km.post, c), km.post2, c);
   }
 }
 
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
   }
   for (int idx = threadIdx.x; idx < nrow * N; idx += blockDim.x) {
     int col = idx / nrow, r = idx - col * nrow;
-    M[(row0 + r) + (size_t)col * N] = apply_fac<R>(tile[col * RB + r], km.post, c);
+    M[(row0 + r) + (size_t)col * N] = apply_fac<R>(apply_fac<R>(tile[col * RB + r], km.post, c), km.post2, c);
   }
 }
 
@@ -169,6 +169,6 @@ __global__ void __launch_bounds__(256) k_kin_elem(KinArgs<R> a) {
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N;
        idx += gridDim.x * blockDim.x) {
     int col = idx / N, r = idx - col * N;
-    M[idx] = apply_fac<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km.post, c);
+    M[idx] = apply_fac<R>(apply_fac<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km.post, c), km.post2, c);
   }
 }

@@ -16,6 +16,7 @@ struct QrpArgs {
   int *ipvt;       // chain stride N
   double *D;       // chain stride N (may alias a propagator D vector)
   int N;
+  int nopiv;       // 1: plain Householder QR (no column exchanges)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -75,7 +76,7 @@ __global__ void __launch_bounds__(1024) k_qrp(QrpArgs a) {
       s_pvt = bj;
     }
     __syncthreads();
-    const int pvt = s_pvt;
+    const int pvt = a.nopiv ? i : s_pvt;
     if (pvt != i) {
       double *cp = A + (size_t)pvt * N, *ci = A + (size_t)i * N;
       for (int r = threadIdx.x; r < N; r += blockDim.x) {
@@ -177,20 +178,19 @@ __global__ void __launch_bounds__(1024) k_qrp(QrpArgs a) {
 // Register-blocked variant of k_qrp / k_orgqr for N <= 64*RPL: every lane
 // keeps its RPL rows of the Householder vector in registers and a wavefront
 // updates CPW columns at once (one load/store round trip per column group).
-#define QR_CPW 8
 #define QR_REG_THREADS 512  // 2 waves per SIMD: room for CPW*RPL doubles per lane
 
 // apply H = I - tau v v^T (v in registers vr[t] for rows r = i + lane + 64 t)
 // to columns j of M[i:, j]; optional ZLAQP2 norm downdating.
-template <int RPL>
+template <int RPL, int CPW>
 __device__ __forceinline__ void apply_h_cols(double *M, int N, int i, const double (&vr)[RPL], double tau,
                                              int jbeg, int jstep, double *vn1, double *vn2, int lane) {
   const double tol3z = 1.4901161193847656e-08;  // sqrt(2^-53)
-  for (int j0 = jbeg; j0 < N; j0 += QR_CPW * jstep) {
-    double x[QR_CPW][RPL];
-    double d[QR_CPW];
+  for (int j0 = jbeg; j0 < N; j0 += CPW * jstep) {
+    double x[CPW][RPL];
+    double d[CPW];
 #pragma unroll
-    for (int cc = 0; cc < QR_CPW; ++cc) {
+    for (int cc = 0; cc < CPW; ++cc) {
       const int j = j0 + cc * jstep;
       d[cc] = 0.0;
 #pragma unroll
@@ -201,15 +201,15 @@ __device__ __forceinline__ void apply_h_cols(double *M, int N, int i, const doub
     }
     if (tau != 0.0) {
 #pragma unroll
-      for (int cc = 0; cc < QR_CPW; ++cc)
+      for (int cc = 0; cc < CPW; ++cc)
 #pragma unroll
         for (int t = 0; t < RPL; ++t) d[cc] += x[cc][t] * vr[t];
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-        for (int cc = 0; cc < QR_CPW; ++cc) d[cc] += __shfl_xor(d[cc], off, 64);
+        for (int cc = 0; cc < CPW; ++cc) d[cc] += __shfl_xor(d[cc], off, 64);
 #pragma unroll
-      for (int cc = 0; cc < QR_CPW; ++cc) {
+      for (int cc = 0; cc < CPW; ++cc) {
         const int j = j0 + cc * jstep;
         const double tt = -tau * d[cc];
 #pragma unroll
@@ -222,7 +222,7 @@ __device__ __forceinline__ void apply_h_cols(double *M, int N, int i, const doub
     }
     if (vn1) {
 #pragma unroll
-      for (int cc = 0; cc < QR_CPW; ++cc) {
+      for (int cc = 0; cc < CPW; ++cc) {
         const int j = j0 + cc * jstep;
         if (j >= N) continue;
         const double aij = __shfl(x[cc][0], 0, 64);  // row i sits in lane 0, t = 0
@@ -254,7 +254,7 @@ __device__ __forceinline__ void apply_h_cols(double *M, int N, int i, const doub
   }
 }
 
-template <int RPL>
+template <int RPL, int CPW>
 __global__ void __launch_bounds__(QR_REG_THREADS) k_qrp_reg(QrpArgs a) {
   const int N = a.N;
   const int chain = blockIdx.x;
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(QR_REG_THREADS) k_qrp_reg(QrpArgs a) {
       s_pvt = bj;
     }
     __syncthreads();
-    const int pvt = s_pvt;
+    const int pvt = a.nopiv ? i : s_pvt;
     double *ci = A + (size_t)i * N;
     if (pvt != i) {
       double *cp = A + (size_t)pvt * N;
@@ -353,7 +353,7 @@ __global__ void __launch_bounds__(QR_REG_THREADS) k_qrp_reg(QrpArgs a) {
       vr[t] = (r == i) ? 1.0 : (r < N ? ci[r] * sc : 0.0);
     }
     __syncthreads();  // every wave holds v before wave 0 overwrites column i below
-    apply_h_cols<RPL>(A, N, i, vr, tau_i, i + 1 + wid, nw, vn1, vn2, lane);
+    apply_h_cols<RPL, CPW>(A, N, i, vr, tau_i, i + 1 + wid, nw, vn1, vn2, lane);
     // the reflector itself (wave 0 owns the write-back of column i)
     if (wid == 0) {
 #pragma unroll
@@ -376,7 +376,7 @@ __global__ void __launch_bounds__(QR_REG_THREADS) k_qrp_reg(QrpArgs a) {
   }
 }
 
-template <int RPL>
+template <int RPL, int CPW>
 __global__ void __launch_bounds__(QR_REG_THREADS) k_orgqr_reg(const double *Ain, const double *tauin, double *Qout,
                                                     int N) {
   const int chain = blockIdx.x;
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(QR_REG_THREADS) k_orgqr_reg(const double *Ain,
       const int r = i + lane + 64 * t;
       vr[t] = (r == i) ? 1.0 : (r < N ? v[r] : 0.0);
     }
-    apply_h_cols<RPL>(Q, N, i, vr, ti, i + 1 + wid, nw, nullptr, nullptr, lane);
+    apply_h_cols<RPL, CPW>(Q, N, i, vr, ti, i + 1 + wid, nw, nullptr, nullptr, lane);
     double *qi = Q + (size_t)i * N;
     for (int r = threadIdx.x; r < N; r += blockDim.x) {
       if (r < i) qi[r] = 0.0;
@@ -454,27 +454,31 @@ __global__ void __launch_bounds__(1024) k_orgqr(const double *Ain, const double 
 struct GemmArgs {
   const double *A, *B;
   double *C;
-  long sA, sB, sC;  // chain strides
-  int N;
+  long sA, sB, sC;      // chain strides
+  int M, Nc, K;         // C is M x Nc, op(A) M x K, op(B) K x Nc
+  int lda, ldb, ldc;
   int ta, tb, triA, triB;
-  double beta;
+  double alpha, beta;
 };
 
 #define GT 64   // tile M, N
 #define GK 16   // tile K
 
-__device__ __forceinline__ double gemm_ld(const double *M, int N, int trans, int tri, int r, int c) {
-  // element (r, c) of op(M)
+// element (r, c) of op(X), op(X) having R rows and Cc columns
+__device__ __forceinline__ double gemm_ld(const double *X, int ld, int trans, int tri, int r, int c, int R,
+                                          int Cc) {
+  if (r >= R || c >= Cc) return 0.0;
   int sr = trans ? c : r, sc = trans ? r : c;
-  if (sr >= N || sc >= N) return 0.0;
   if (tri && sr > sc) return 0.0;
-  return M[sr + (size_t)sc * N];
+  return X[sr + (size_t)sc * ld];
 }
 
+// C = alpha op(A) op(B) + beta C, batched over chains (grid.z), FP64 MFMA 16x16x4.
+// trans flags: 0 = N, 1 = T.  triA / triB: use only the upper triangle (row <= col,
+// in stored coordinates) of A / B (ZTRMM as a masked GEMM).
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   __shared__ double As[GK][GT + 1];  // As[k][m]
   __shared__ double Bs[GK][GT + 1];  // Bs[k][n]
-  const int N = g.N;
   const int chain = blockIdx.z;
   const double *A = g.A + chain * g.sA;
   const double *B = g.B + chain * g.sB;
@@ -486,12 +490,11 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   d4 acc[2][2];
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) acc[x][y] = (d4){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < N; k0 += GK) {
+  for (int k0 = 0; k0 < g.K; k0 += GK) {
     for (int idx = threadIdx.x; idx < GT * GK; idx += blockDim.x) {
       int m = idx % GT, k = idx / GT;
-      As[k][m] = gemm_ld(A, N, g.ta, g.triA, m0 + m, k0 + k);
-      int n = idx % GT, kk = idx / GT;
-      Bs[kk][n] = gemm_ld(B, N, g.tb, g.triB, k0 + kk, n0 + n);
+      As[k][m] = gemm_ld(A, g.lda, g.ta, g.triA, m0 + m, k0 + k, g.M, g.K);
+      Bs[k][m] = gemm_ld(B, g.ldb, g.tb, g.triB, k0 + k, n0 + m, g.K, g.Nc);
     }
     __syncthreads();
     for (int kk = 0; kk < GK; kk += 4) {
@@ -511,9 +514,9 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
         // v_mfma_f64_16x16x4f64 result: lane l, item q holds D[4q + l/16][l%16]
         int row = m0 + wm + 16 * x + 4 * q + (lane >> 4);
         int col = n0 + wn + 16 * y + (lane & 15);
-        if (row < N && col < N) {
-          double v = acc[x][y][q];
-          size_t o = row + (size_t)col * N;
+        if (row < g.M && col < g.Nc) {
+          double v = g.alpha * acc[x][y][q];
+          size_t o = row + (size_t)col * g.ldc;
           C[o] = (g.beta != 0.0) ? v + g.beta * C[o] : v;
         }
       }

@@ -14,6 +14,7 @@ struct ChainConst {
   double boson[9];        // exp(alpha*dS) [* exp(gamma*dT)], index (dS/4+1)*3 + (dT/4+1)
   double ep, em;          // exp(+dtau*mu), exp(-dtau*mu)
   double beta;
+  double rt;              // hopping t (current-current correlator prefactor)
 };
 
 // lattice tables shared by all chains of one context (same L)

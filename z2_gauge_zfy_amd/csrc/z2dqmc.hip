@@ -189,6 +189,7 @@ static ChainConst make_const(const z2_chain_desc &d, int Ltrot) {
   k.ep = exp(Dtau * d.mu);
   k.em = exp(-Dtau * d.mu);
   k.beta = d.Beta;
+  k.rt = d.RT;
   return k;
 }
 
@@ -625,6 +626,7 @@ static KinMat<R> kmat(R *m, int nflag, int flush = 0, int pre1 = 0, int pre2 = 0
   k.mid1 = mid1;
   k.mid2 = mid2;
   k.post = post;
+  k.post2 = 0;
   k.pre_cond = 0;
   return k;
 }
@@ -755,20 +757,30 @@ static int elem(z2_ctx *c, const double *in, double *out, int mode, const double
   return Z2_OK;
 }
 
-static int gemm(z2_ctx *c, const double *A, int ta, const double *B, int tb, double *C, int triA = 0,
-                int triB = 0) {
+// general batched GEMM: C(M x Nc) = alpha op(A) op(B) + beta C, chain strides sA/sB/sC
+static int gemm_g(z2_ctx *c, int M, int Nc, int K, const double *A, int lda, long sA, int ta,
+                  const double *B, int ldb, long sB, int tb, double *C, int ldc, long sC, double alpha = 1.0,
+                  double beta = 0.0, int triA = 0, int triB = 0) {
   GemmArgs g;
   g.A = A;
   g.B = B;
   g.C = C;
-  g.sA = g.sB = g.sC = (long)c->N * c->N;
-  g.N = c->N;
+  g.sA = sA;
+  g.sB = sB;
+  g.sC = sC;
+  g.M = M;
+  g.Nc = Nc;
+  g.K = K;
+  g.lda = lda;
+  g.ldb = ldb;
+  g.ldc = ldc;
   g.ta = ta;
   g.tb = tb;
   g.triA = triA;
   g.triB = triB;
-  g.beta = 0.0;
-  dim3 grid((c->N + GT - 1) / GT, (c->N + GT - 1) / GT, c->nch);
+  g.alpha = alpha;
+  g.beta = beta;
+  dim3 grid((M + GT - 1) / GT, (Nc + GT - 1) / GT, c->nch);
   CHK(prof_begin(c, Z2_K_GEMM));
   hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, c->stream, g);
   LAUNCH_CHECK();
@@ -776,12 +788,25 @@ static int gemm(z2_ctx *c, const double *A, int ta, const double *B, int tb, dou
   return Z2_OK;
 }
 
-// register-blocked QR kernels for N <= 512, the plain ones above
+// square N x N GEMM on per-chain matrices
+static int gemm(z2_ctx *c, const double *A, int ta, const double *B, int tb, double *C, int triA = 0,
+                int triB = 0) {
+  const int N = c->N;
+  const long nn = (long)N * N;
+  return gemm_g(c, N, N, N, A, N, nn, ta, B, N, nn, tb, C, N, nn, 1.0, 0.0, triA, triB);
+}
+
+// register-blocked QR kernels for N <= 1024 (RPL = ceil(N/64) rows per lane,
+// CPW columns per wavefront pass, sized to stay within 256 VGPRs), plain ones above
+#define QR_CASES(X)                                                                              \
+  X(1, 8) X(2, 8) X(3, 8) X(4, 8) X(5, 4) X(6, 4) X(7, 4) X(8, 4) X(9, 2) X(10, 2) X(11, 2) X(12, 2) \
+  X(13, 2) X(14, 2) X(15, 2) X(16, 2)
+
 static int launch_qrp(int N, dim3 grid, size_t sh, hipStream_t st, QrpArgs q) {
   const int rpl = (N + 63) / 64;
   switch (rpl) {
-#define QCASE(R) case R: hipLaunchKernelGGL((k_qrp_reg<R>), grid, dim3(QR_REG_THREADS), sh, st, q); break;
-    QCASE(1) QCASE(2) QCASE(3) QCASE(4) QCASE(5) QCASE(6) QCASE(7) QCASE(8)
+#define QCASE(R, W) case R: hipLaunchKernelGGL((k_qrp_reg<R, W>), grid, dim3(QR_REG_THREADS), sh, st, q); break;
+    QR_CASES(QCASE)
 #undef QCASE
     default: hipLaunchKernelGGL(k_qrp, grid, dim3(1024), sh, st, q);
   }
@@ -792,8 +817,8 @@ static int launch_qrp(int N, dim3 grid, size_t sh, hipStream_t st, QrpArgs q) {
 static int launch_orgqr(int N, dim3 grid, hipStream_t st, const double *A, const double *tau, double *Q) {
   const int rpl = (N + 63) / 64;
   switch (rpl) {
-#define OCASE(R) case R: hipLaunchKernelGGL((k_orgqr_reg<R>), grid, dim3(QR_REG_THREADS), 0, st, A, tau, Q, N); break;
-    OCASE(1) OCASE(2) OCASE(3) OCASE(4) OCASE(5) OCASE(6) OCASE(7) OCASE(8)
+#define OCASE(R, W) case R: hipLaunchKernelGGL((k_orgqr_reg<R, W>), grid, dim3(QR_REG_THREADS), 0, st, A, tau, Q, N); break;
+    QR_CASES(OCASE)
 #undef OCASE
     default: hipLaunchKernelGGL(k_orgqr, grid, dim3(1024), 0, st, A, tau, Q, N);
   }
@@ -808,6 +833,7 @@ static int qrp(z2_ctx *c, double *A, double *D) {
   q.ipvt = c->ipvt;
   q.D = D;
   q.N = c->N;
+  q.nopiv = 0;
   size_t sh = sizeof(double) * (2 * c->N + 32) + sizeof(int) * 32;
   CHK(prof_begin(c, Z2_K_QRP));
   CHK(launch_qrp(c->N, dim3(c->nch), sh, c->stream, q));
@@ -1217,8 +1243,9 @@ extern "C" int z2_test_gemm(int32_t n, const double *A, int32_t ta, const double
   HIPC(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(dB, B, nn * 8, hipMemcpyHostToDevice));
   GemmArgs g;
-  g.A = dA; g.B = dB; g.C = dC; g.sA = g.sB = g.sC = (long)nn; g.N = n;
-  g.ta = ta; g.tb = tb; g.triA = triA; g.triB = triB; g.beta = 0.0;
+  g.A = dA; g.B = dB; g.C = dC; g.sA = g.sB = g.sC = (long)nn; g.M = g.Nc = g.K = n;
+  g.lda = g.ldb = g.ldc = n;
+  g.ta = ta; g.tb = tb; g.triA = triA; g.triB = triB; g.alpha = 1.0; g.beta = 0.0;
   dim3 grid((n + GT - 1) / GT, (n + GT - 1) / GT, 1);
   hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, 0, g);
   LAUNCH_CHECK();
@@ -1236,7 +1263,7 @@ extern "C" int z2_test_qrp(int32_t n, double *A, double *D, int32_t *ipvt, doubl
   if (!dA || !dD || !dt || !dp || !dQ) return set_err(Z2_ENOMEM, "alloc");
   HIPC(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
   QrpArgs q;
-  q.A = dA; q.tau = dt; q.ipvt = dp; q.D = dD; q.N = n;
+  q.A = dA; q.tau = dt; q.ipvt = dp; q.D = dD; q.N = n; q.nopiv = 0;
   size_t sh = sizeof(double) * (2 * n + 32) + sizeof(int) * 32;
   CHK(launch_qrp(n, dim3(1), sh, 0, q));
   if (Q) CHK(launch_orgqr(n, dim3(1), 0, (const double *)dA, (const double *)dt, dQ));
