@@ -73,6 +73,27 @@ int z2_pre(z2_ctx *ctx);
  * with equal-time observables at every slice; is_beta selects the pass order. */
 int z2_sweep(z2_ctx *ctx, int32_t is_beta);
 
+/* time-displaced measurement (is_tau = 1 in the reference's parameter file):
+ * z2_enable_tau allocates its state (about 26 N^2 doubles per chain);
+ * z2_sweep_tau(ctx, is_beta, toggle) = LocalSweep%sweep with toggle
+ * (src/local_sweep.f90:152-195): with toggle, Dyn%reset + Dyn%sweep_R
+ * (src/dynamics.f90:41-62) run at the start of every right pass, and the
+ * Obs_tau accumulators (src/obser_tau.f90:56-133) are divided by the number of
+ * such passes (Obs_tau%ave, src/obser_tau.f90:37-47).  z2_sweep(ctx, b) is
+ * z2_sweep_tau(ctx, b, 0). */
+int z2_enable_tau(z2_ctx *ctx);
+int z2_sweep_tau(z2_ctx *ctx, int32_t is_beta, int32_t toggle);
+/* Dyn%reset + Dyn%sweep_R on their own from the current state, accumulators
+ * zeroed first and not normalised (unit step of the measurement) */
+int z2_dyn_measure(z2_ctx *ctx);
+/* time-displaced correlators of one chain, out[7 * Ltrot * N] laid out
+ * [k][nt-1][d]: k = spin, pair_s, pair_d, charge_s, charge_d, single, curxx
+ * (Obs_tau fields, src/obser_tau.f90:6-16), d = imj(i,j) - 1 the displacement */
+int z2_get_obs_tau(z2_ctx *ctx, int32_t chain, double *out);
+/* Wrap_tau drift of G(0,t), G(t,0), G(t,t): running max and running sum of the
+ * per-recompute maxima (src/stabilization.f90:496-508) */
+int z2_get_dyn_drift(z2_ctx *ctx, int32_t chain, double xmax[3], double xmean[3]);
+
 /* One sweep of ctx in lockstep with an FP64 reference context of the same
  * shape (no counterpart in the reference; measures the precision deviation):
  * ctx replays ref's accept/reject decisions, and maxdiff[chain] receives
@@ -121,6 +142,7 @@ int z2_get_stream(z2_ctx *ctx, void **stream);
 #define Z2_K_ORGQR 6
 #define Z2_K_TRSM 7
 #define Z2_K_OBS 8
+#define Z2_K_OBS_TAU 9
 int z2_profile_kernel(z2_ctx *ctx, int32_t kind);
 int z2_profile_read(z2_ctx *ctx, double *total_ms, int64_t *launches);
 

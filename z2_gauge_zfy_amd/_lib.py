@@ -17,7 +17,7 @@ Z2_PREC_MIXED = 1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
-              "obs": 8}
+              "obs": 8, "obs_tau": 9}
 
 
 class Z2Params(C.Structure):
@@ -51,6 +51,12 @@ _SIGS = {
     "z2_get_seed": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "z2_pre": (C.c_int, [C.c_void_p]),
     "z2_sweep": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_enable_tau": (C.c_int, [C.c_void_p]),
+    "z2_sweep_tau": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_dyn_measure": (C.c_int, [C.c_void_p]),
+    "z2_get_obs_tau": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_get_dyn_drift": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
     "z2_sweep_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_wrap_pre": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_wrap_L": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),

@@ -528,18 +528,18 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 // mode 0 (ZTRSM 'L','U','N','N'): x_k = b_k / R_kk, then b_i -= x_k R_ik.
 // mode 1 (ZTRSM 'R','U','C','N' applied to each row): x_k = (1/R_kk) * b_k, then
 //        b_i -= R_ik x_k.
-// A workgroup owns CBT right-hand sides, staged in LDS.
+// A workgroup owns cbt (<= CBT) right-hand sides, staged in LDS.
 #define CBT 16
 __global__ void __launch_bounds__(256) k_trsm(const double *Rm, double *X, int N, int rs, int cs,
-                                              int mode) {
+                                              int mode, int cbt) {
   extern __shared__ unsigned char smem_raw[];
-  double *xb = reinterpret_cast<double *>(smem_raw);  // [CBT][N]
-  double *rk = xb + CBT * N;                          // column k of R
+  double *xb = reinterpret_cast<double *>(smem_raw);  // [cbt][N]
+  double *rk = xb + cbt * N;                          // column k of R
   const int chain = blockIdx.y;
   const double *R = Rm + (size_t)chain * N * N;
   double *Xc = X + (size_t)chain * N * N;
-  const int c0 = blockIdx.x * CBT;
-  const int nc = min(CBT, N - c0);
+  const int c0 = blockIdx.x * cbt;
+  const int nc = min(cbt, N - c0);
   for (int idx = threadIdx.x; idx < nc * N; idx += blockDim.x) {
     int c = idx / N, i = idx - c * N;
     xb[c * N + i] = Xc[(size_t)i * rs + (size_t)(c0 + c) * cs];

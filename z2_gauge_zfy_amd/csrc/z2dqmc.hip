@@ -16,6 +16,7 @@
 #include "kin_kernels.hpp"
 #include "linalg_kernels.hpp"
 #include "obs_kernels.hpp"
+#include "tau_kernels.hpp"
 
 static thread_local std::string g_err;
 
@@ -93,6 +94,16 @@ struct z2_ctx {
   const uint32_t *force = nullptr;
   int dwords = 0;
   double *cmp = nullptr;  // per-chain max |G - G_other| (shadow runs)
+  // time-displaced measurement (Dyn%sweep_R): its own right propagator d*, the four
+  // Green's functions G(0,0), G(0,t), G(t,0), G(t,t), the 2N x 2N recompute scratch
+  bool tau_on = false;
+  double *dUUR = nullptr, *dVUR = nullptr, *dDUR = nullptr, *dUUL = nullptr, *dVUL = nullptr, *dDUL = nullptr;
+  double *G00 = nullptr, *G0t = nullptr, *Gt0 = nullptr, *Gtt = nullptr;
+  double *iVR = nullptr, *iVL = nullptr;
+  double *bigA = nullptr, *bigQ = nullptr, *bigZ = nullptr, *bigR = nullptr, *tau2 = nullptr, *D2 = nullptr;
+  int *ipvt2 = nullptr;
+  double *otau = nullptr;       // [chain][7][T][N]
+  double *dyn_drift = nullptr;  // [chain][6]: max of G0t, Gt0, Gtt drifts, then their sums
   std::vector<void *> allocs;
   bool have_fields = false, have_pre = false;
   // kernel timing hooks: HIP events around every launch of one kernel class
@@ -826,35 +837,52 @@ static int launch_orgqr(int N, dim3 grid, hipStream_t st, const double *A, const
   return Z2_OK;
 }
 
-static int qrp(z2_ctx *c, double *A, double *D) {
+// QR of the chains' n x n matrices A (column-pivoted unless nopiv): R with rows
+// scaled to unit diagonal magnitude in the upper triangle, D = |diag R|
+static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n, int nopiv) {
   QrpArgs q;
   q.A = A;
-  q.tau = c->tau;
-  q.ipvt = c->ipvt;
+  q.tau = tau;
+  q.ipvt = ipvt;
   q.D = D;
-  q.N = c->N;
-  q.nopiv = 0;
-  size_t sh = sizeof(double) * (2 * c->N + 32) + sizeof(int) * 32;
+  q.N = n;
+  q.nopiv = nopiv;
+  size_t sh = sizeof(double) * (2 * n + 32) + sizeof(int) * 32;
   CHK(prof_begin(c, Z2_K_QRP));
-  CHK(launch_qrp(c->N, dim3(c->nch), sh, c->stream, q));
+  CHK(launch_qrp(n, dim3(c->nch), sh, c->stream, q));
   CHK(prof_end(c, Z2_K_QRP));
   return Z2_OK;
 }
 
-static int orgqr(z2_ctx *c, const double *A, double *Qo) {
+static int qrp(z2_ctx *c, double *A, double *D) { return qrp_n(c, A, D, c->tau, c->ipvt, c->N, 0); }
+
+static int orgqr_n(z2_ctx *c, const double *A, const double *tau, double *Qo, int n) {
   CHK(prof_begin(c, Z2_K_ORGQR));
-  CHK(launch_orgqr(c->N, dim3(c->nch), c->stream, A, (const double *)c->tau, Qo));
+  CHK(launch_orgqr(n, dim3(c->nch), c->stream, A, tau, Qo));
   CHK(prof_end(c, Z2_K_ORGQR));
   return Z2_OK;
 }
 
-static int trsm(z2_ctx *c, const double *R, double *X, int right) {
-  dim3 grid((c->N + CBT - 1) / CBT, c->nch);
-  size_t sh = sizeof(double) * (CBT + 1) * c->N;
-  CHK(prof_begin(c, Z2_K_TRSM));
-  if (right) hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, c->stream, R, X, c->N, c->N, 1, 1);
-  else hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, c->stream, R, X, c->N, 1, c->N, 0);
+static int orgqr(z2_ctx *c, const double *A, double *Qo) { return orgqr_n(c, A, c->tau, Qo, c->N); }
+
+// n x n triangular solves for nb matrices; the right-hand-side block shrinks so that
+// (cbt + 1) n doubles fit the LDS of one workgroup
+static int launch_trsm(int n, int nb, hipStream_t st, size_t lds_max, const double *R, double *X, int right) {
+  int cbt = CBT;
+  while (cbt > 1 && sizeof(double) * (size_t)(cbt + 1) * n > lds_max) --cbt;
+  size_t sh = sizeof(double) * (size_t)(cbt + 1) * n;
+  if (sh > lds_max) return set_err(Z2_EINVAL, "triangular solve of order %d exceeds the LDS", n);
+  if (sh > 65536) HIPC(hipFuncSetAttribute((const void *)k_trsm, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  dim3 grid((n + cbt - 1) / cbt, nb);
+  if (right) hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, st, R, X, n, n, 1, 1, cbt);
+  else hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, st, R, X, n, 1, n, 0, cbt);
   LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+static int trsm(z2_ctx *c, const double *R, double *X, int right, int n = 0) {
+  CHK(prof_begin(c, Z2_K_TRSM));
+  CHK(launch_trsm(n ? n : c->N, c->nch, c->stream, c->lds_max, R, X, right));
   CHK(prof_end(c, Z2_K_TRSM));
   return Z2_OK;
 }
@@ -865,15 +893,17 @@ static int dcopy(z2_ctx *c, double *dst, const double *src, size_t n) {
 }
 
 // ---------------------------------------------------------------- stabilization
-// stab_UR, src/stabilization.f90:70-94
-static int stab_UR(z2_ctx *c) {
-  CHK(elem(c, c->UUR, c->M, 2, c->DUR));
-  CHK(qrp(c, c->M, c->DUR));
-  CHK(elem(c, c->VUR, c->X, 4, nullptr, c->ipvt));
-  CHK(gemm(c, c->M, 0, c->X, 0, c->VUR, 1, 0));
-  CHK(orgqr(c, c->M, c->UUR));
+// stab_UR, src/stabilization.f90:70-94, on the right propagator (U, D, V)
+static int stab_UR_buf(z2_ctx *c, double *U, double *D, double *V) {
+  CHK(elem(c, U, c->M, 2, D));
+  CHK(qrp(c, c->M, D));
+  CHK(elem(c, V, c->X, 4, nullptr, c->ipvt));
+  CHK(gemm(c, c->M, 0, c->X, 0, V, 1, 0));
+  CHK(orgqr(c, c->M, U));
   return Z2_OK;
 }
+
+static int stab_UR(z2_ctx *c) { return stab_UR_buf(c, c->UUR, c->DUR, c->VUR); }
 
 // stab_UL, src/stabilization.f90:96-120
 static int stab_UL(z2_ctx *c) {
@@ -1076,6 +1106,177 @@ static int obs_ave(z2_ctx *c) {
   return Z2_OK;
 }
 
+// ---------------------------------------------------------------- time-displaced
+extern "C" int z2_enable_tau(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (c->tau_on) return Z2_OK;
+  const size_t nn = (size_t)c->N * c->N, B = (size_t)c->nch, n2n2 = 4 * nn;
+  double **mats[] = {&c->dUUR, &c->dVUR, &c->dUUL, &c->dVUL, &c->G00, &c->G0t, &c->Gt0, &c->Gtt, &c->iVR, &c->iVL};
+  for (double **m : mats) CHK(dalloc(c, m, B * nn));
+  for (double **m : {&c->bigA, &c->bigQ, &c->bigZ, &c->bigR}) CHK(dalloc(c, m, B * n2n2));
+  for (double **d : {&c->dDUR, &c->dDUL}) CHK(dalloc(c, d, B * c->N));
+  CHK(dalloc(c, &c->tau2, B * 2 * c->N));
+  CHK(dalloc(c, &c->D2, B * 2 * c->N));
+  CHK(dalloc(c, &c->ipvt2, B * 2 * c->N));
+  CHK(dalloc(c, &c->otau, B * TAU_NOBS * c->T * c->N));
+  CHK(dalloc(c, &c->dyn_drift, B * 6));
+  HIPC(hipMemsetAsync(c->otau, 0, sizeof(double) * B * TAU_NOBS * c->T * c->N, c->stream));
+  HIPC(hipMemsetAsync(c->dyn_drift, 0, sizeof(double) * B * 6, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->tau_on = true;
+  return Z2_OK;
+}
+
+static dim3 ew_grid(const z2_ctx *c, size_t n) {
+  size_t nb = (n + 255) / 256;
+  return dim3((unsigned)(nb > 64 ? 64 : nb), c->nch);
+}
+
+// Ainv <- A^{-1} for each chain's N x N A: A P = Q D R (pivoted QR, R unit-diagonal
+// magnitude), so A^{-1} = P R^{-1} D^{-1} Q^T.  Scratch: M, Q, X, Dtmp, tau, ipvt.
+static int ginv(z2_ctx *c, const double *A, double *Ainv) {
+  CHK(elem(c, A, c->M, 0));
+  CHK(qrp(c, c->M, c->Dtmp));
+  CHK(orgqr(c, c->M, c->Q));
+  CHK(elem(c, c->Q, c->X, 1));
+  CHK(elem(c, c->X, c->X, 8, c->Dtmp));
+  CHK(trsm(c, c->M, c->X, 0));
+  CHK(elem(c, c->X, Ainv, 6, nullptr, c->ipvt));
+  return Z2_OK;
+}
+
+// stab_green_big, src/stabilization.f90:202-264, on the d* propagators -> bigZ (2N x 2N)
+//   big = [[ (V_R V_L)^{-1}, D_L ], [ -D_R, (U_L U_R)^{-1} ]] = Qb Db Rb (plain QR)
+//   G_big = diag(V_R^{-1}, U_L^{-1}) Rb^{-1} Db^{-1} . Qb^T diag(V_L^{-1}, U_R^{-1})
+// U_R, U_L are orthogonal (explicit Householder Q), so their inverses are transposes.
+static int stab_green_big(z2_ctx *c) {
+  const int N = c->N, N2 = 2 * N;
+  const long nn = (long)N * N, n4 = 4 * nn;
+  CHK(ginv(c, c->dVUR, c->iVR));
+  CHK(ginv(c, c->dVUL, c->iVL));
+  CHK(gemm(c, c->iVL, 0, c->iVR, 0, c->Y));     // (V_R V_L)^{-1}
+  CHK(gemm(c, c->dUUR, 1, c->dUUL, 1, c->X));   // (U_L U_R)^{-1}
+  {
+    dim3 g = ew_grid(c, (size_t)n4);
+    hipLaunchKernelGGL(k_assemble_big, g, dim3(256), 0, c->stream, (const double *)c->Y, (const double *)c->X,
+                       (const double *)c->dDUL, (const double *)c->dDUR, c->bigA, N);
+    LAUNCH_CHECK();
+  }
+  CHK(qrp_n(c, c->bigA, c->D2, c->tau2, c->ipvt2, N2, 1));
+  CHK(orgqr_n(c, c->bigA, c->tau2, c->bigQ, N2));
+  // right = Qb^T diag(V_L^{-1}, U_R^T)
+  CHK(gemm_g(c, N2, N, N, c->bigQ, N2, n4, 1, c->iVL, N, nn, 0, c->bigR, N2, n4));
+  CHK(gemm_g(c, N2, N, N, c->bigQ + N, N2, n4, 1, c->dUUR, N, nn, 1, c->bigR + (size_t)N * N2, N2, n4));
+  // Z = Rb^{-1}
+  {
+    dim3 g = ew_grid(c, (size_t)n4);
+    hipLaunchKernelGGL(k_eye, g, dim3(256), 0, c->stream, c->bigZ, N2);
+    LAUNCH_CHECK();
+  }
+  CHK(trsm(c, c->bigA, c->bigZ, 0, N2));
+  // left = diag(V_R^{-1}, U_L^T) Z Db^{-1}  (into bigQ)
+  CHK(gemm_g(c, N, N2, N, c->iVR, N, nn, 0, c->bigZ, N2, n4, 0, c->bigQ, N2, n4));
+  CHK(gemm_g(c, N, N2, N, c->dUUL, N, nn, 1, c->bigZ + N, N2, n4, 0, c->bigQ + N, N2, n4));
+  {
+    dim3 g = ew_grid(c, (size_t)n4);
+    hipLaunchKernelGGL(k_div_cols, g, dim3(256), 0, c->stream, c->bigQ, (const double *)c->D2, N2);
+    LAUNCH_CHECK();
+  }
+  CHK(gemm_g(c, N2, N2, N2, c->bigQ, N2, n4, 0, c->bigR, N2, n4, 0, c->bigZ, N2, n4));
+  return Z2_OK;
+}
+
+// Wrap_tau, src/stabilization.f90:478-514
+static int wrap_tau(z2_ctx *c, int nt) {
+  const int st = nt / c->w;
+  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
+  CHK(dcopy(c, c->dUUL, c->ULl + slot(c, st), nn));
+  CHK(dcopy(c, c->dVUL, c->VLl + slot(c, st), nn));
+  CHK(dcopy(c, c->dDUL, c->DLl + slotd(c, st), nd));
+  CHK(stab_UR_buf(c, c->dUUR, c->dDUR, c->dVUR));
+  CHK(stab_green_big(c));
+  hipLaunchKernelGGL(k_tau_replace, dim3(c->nch), dim3(256), 0, c->stream, (const double *)c->bigZ, c->G00,
+                     c->G0t, c->Gt0, c->Gtt, c->dyn_drift, c->N);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+static int obs_tau(z2_ctx *c, int nt) {
+  TauObsArgs o;
+  o.G00 = c->G00; o.Gtt = c->Gtt; o.Gt0 = c->Gt0; o.G0t = c->G0t;
+  o.sig = c->sig; o.acc = c->otau; o.cc = c->dcc;
+  o.T = c->T; o.nt = nt; o.Lx = c->Lx; o.Ly = c->Ly; o.lat = c->lat;
+  const int th = ((c->N + 63) / 64) * 64;
+  CHK(prof_begin(c, Z2_K_OBS_TAU));
+  hipLaunchKernelGGL(k_obs_tau, dim3(c->nch), dim3(th), 0, c->stream, o);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_OBS_TAU));
+  return Z2_OK;
+}
+
+// Dyn_reset + Dyn%sweep_R, src/dynamics.f90:41-62: starts from the current G and
+// right propagator, steps G(t,0), G(0,t), G(t,t) and U_R through all slices.
+static int dyn_sweep(z2_ctx *c) {
+  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
+  CHK(dcopy(c, c->dUUR, c->UUR, nn));
+  CHK(dcopy(c, c->dVUR, c->VUR, nn));
+  CHK(dcopy(c, c->dDUR, c->DUR, nd));
+  {
+    dim3 g = ew_grid(c, (size_t)c->N * c->N);
+    if (c->mixed)
+      hipLaunchKernelGGL((k_dyn_reset<float>), g, dim3(256), 0, c->stream, (const float *)c->Gf, c->G00, c->Gtt,
+                         c->Gt0, c->G0t, c->N);
+    else
+      hipLaunchKernelGGL((k_dyn_reset<double>), g, dim3(256), 0, c->stream, (const double *)c->G, c->G00, c->Gtt,
+                         c->Gt0, c->G0t, c->N);
+    LAUNCH_CHECK();
+  }
+  for (int nt = 1; nt <= c->T; ++nt) {
+    CHK(obs_tau(c, nt));
+    KinArgs<double> a = kin_args<double>(c, nt);
+    a.m[0] = kmat<double>(c->Gt0, +1, 0, 0, 0, 0, 0, FAC_EP);   // G(t,0) <- (B G(t,0)) ep
+    a.m[1] = kmat<double>(c->Gtt, +1);                          // G(t,t) <- B G(t,t)
+    a.m[2] = kmat<double>(c->dUUR, +1, 0, 0, 0, 0, 0, FAC_EP);  // U_R <- (B U_R) ep
+    CHK(launch_rows<double>(c, a, 3));
+    a.m[0] = kmat<double>(c->G0t, -1, 0, 0, 0, 0, 0, FAC_EM);   // G(0,t) <- (G(0,t) B^-1) em
+    a.m[1] = kmat<double>(c->Gtt, -1, 0, 0, 0, 0, 0, FAC_EP);   // G(t,t) <- ((G(t,t) B^-1) ep) em
+    a.m[1].post2 = FAC_EM;
+    CHK(launch_cols<double>(c, a, 2));
+    if (nt % c->w == 0) CHK(wrap_tau(c, nt));
+  }
+  return Z2_OK;
+}
+
+extern "C" int z2_dyn_measure(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
+  if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_dyn_measure");
+  HIPC(hipMemsetAsync(c->otau, 0, sizeof(double) * c->nch * TAU_NOBS * c->T * c->N, c->stream));
+  return dyn_sweep(c);
+}
+
+extern "C" int z2_get_obs_tau(z2_ctx *c, int32_t chain, double *out) {
+  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
+  const size_t n = (size_t)TAU_NOBS * c->T * c->N;
+  HIPC(hipMemcpyAsync(out, c->otau + chain * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return Z2_OK;
+}
+
+extern "C" int z2_get_dyn_drift(z2_ctx *c, int32_t chain, double xmax[3], double xmean[3]) {
+  if (!c || !xmax || !xmean || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
+  double v[6];
+  HIPC(hipMemcpyAsync(v, c->dyn_drift + 6 * chain, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 3; ++k) {
+    xmax[k] = v[k];
+    xmean[k] = v[3 + k];
+  }
+  return Z2_OK;
+}
+
 // ---------------------------------------------------------------- drivers
 // Local_sweep_reset's h ~ 0 synchronisation (src/local_sweep.f90:63-69)
 static int h_zero_sync(z2_ctx *c) {
@@ -1108,7 +1309,7 @@ extern "C" int z2_pre(z2_ctx *c) {
 
 // One sweep as a list of steps (src/local_sweep.f90:106-150), so that two
 // contexts can be advanced in lockstep.
-enum StepKind { ST_WRAP_L, ST_WRAP_R, ST_OBS, ST_SLICE_L, ST_SLICE_R };
+enum StepKind { ST_WRAP_L, ST_WRAP_R, ST_OBS, ST_SLICE_L, ST_SLICE_R, ST_DYN };
 struct Step {
   StepKind kind;
   int nt;
@@ -1123,7 +1324,8 @@ static void steps_L(const z2_ctx *c, std::vector<Step> &v) {
   v.push_back({ST_WRAP_L, 0});
 }
 
-static void steps_R(const z2_ctx *c, std::vector<Step> &v) {
+static void steps_R(const z2_ctx *c, std::vector<Step> &v, int toggle) {
+  if (toggle) v.push_back({ST_DYN, 0});
   v.push_back({ST_WRAP_R, 0});
   for (int nt = 1; nt <= c->T; ++nt) {
     v.push_back({ST_SLICE_R, nt});
@@ -1132,13 +1334,13 @@ static void steps_R(const z2_ctx *c, std::vector<Step> &v) {
   }
 }
 
-static void sweep_steps(const z2_ctx *c, int is_beta, std::vector<Step> &v) {
+static void sweep_steps(const z2_ctx *c, int is_beta, int toggle, std::vector<Step> &v) {
   for (int s = 0; s < c->p.Nsweep; ++s) {
     if (is_beta) {
       steps_L(c, v);
-      steps_R(c, v);
+      steps_R(c, v, toggle);
     } else {
-      steps_R(c, v);
+      steps_R(c, v, toggle);
       steps_L(c, v);
     }
   }
@@ -1151,6 +1353,7 @@ static int run_step(z2_ctx *c, const Step &st) {
     case ST_OBS: return z2_obs_equal_calc(c, st.nt);
     case ST_SLICE_L: return z2_slice_L(c, st.nt);
     case ST_SLICE_R: return z2_slice_R(c, st.nt);
+    case ST_DYN: return dyn_sweep(c);
   }
   return set_err(Z2_EINVAL, "bad step");
 }
@@ -1160,17 +1363,33 @@ static bool step_recomputes(const z2_ctx *c, const Step &st) {
   return (st.kind == ST_WRAP_L && st.nt != c->T) || (st.kind == ST_WRAP_R && st.nt != 0);
 }
 
-extern "C" int z2_sweep(z2_ctx *c, int32_t is_beta) {
+// Local_sweep with toggle (src/local_sweep.f90:73-150): with toggle the
+// time-displaced observables are reset, measured at the start of every right
+// pass and divided by the number of such passes.
+extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_sweep");
+  if (toggle && !c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
   CHK(h_zero_sync(c));
   CHK(z2_obs_reset(c));
+  const size_t ntau = (size_t)c->nch * TAU_NOBS * c->T * c->N;
+  if (toggle) HIPC(hipMemsetAsync(c->otau, 0, sizeof(double) * ntau, c->stream));
   std::vector<Step> v;
-  sweep_steps(c, is_beta, v);
-  for (const Step &st : v) CHK(run_step(c, st));
+  sweep_steps(c, is_beta, toggle, v);
+  int nobst = 0;
+  for (const Step &st : v) {
+    CHK(run_step(c, st));
+    nobst += st.kind == ST_DYN;
+  }
   CHK(obs_ave(c));
+  if (toggle && nobst > 0) {
+    hipLaunchKernelGGL(k_scale, dim3(256), dim3(256), 0, c->stream, c->otau, ntau, 1.0 / (double)nobst);
+    LAUNCH_CHECK();
+  }
   return Z2_OK;
 }
+
+extern "C" int z2_sweep(z2_ctx *c, int32_t is_beta) { return z2_sweep_tau(c, is_beta, 0); }
 
 template <typename Rb>
 static int maxdiff(z2_ctx *ref, z2_ctx *c, const Rb *Gc) {
@@ -1192,7 +1411,7 @@ extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *
   CHK(z2_obs_reset(ref));
   HIPC(hipMemsetAsync(c->cmp, 0, sizeof(double) * c->nch, c->stream));
   std::vector<Step> v;
-  sweep_steps(ref, is_beta, v);
+  sweep_steps(ref, is_beta, 0, v);
   c->force = ref->rec;
   int rc = Z2_OK;
   for (const Step &st : v) {
@@ -1284,11 +1503,11 @@ extern "C" int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right
   if (!dR || !dX) return set_err(Z2_ENOMEM, "alloc");
   HIPC(hipMemcpy(dR, R, nn * 8, hipMemcpyHostToDevice));
   HIPC(hipMemcpy(dX, X, nn * 8, hipMemcpyHostToDevice));
-  dim3 grid((n + CBT - 1) / CBT, 1);
-  size_t sh = sizeof(double) * (CBT + 1) * n;
-  if (right) hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, 0, (const double *)dR, dX, n, n, 1, 1);
-  else hipLaunchKernelGGL(k_trsm, grid, dim3(256), sh, 0, (const double *)dR, dX, n, 1, n, 0);
-  LAUNCH_CHECK();
+  int dev = 0, lds = 65536;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds <= 0)
+    lds = 65536;
+  CHK(launch_trsm(n, 1, 0, (size_t)lds, dR, dX, right));
   HIPC(hipDeviceSynchronize());
   HIPC(hipMemcpy(X, dX, nn * 8, hipMemcpyDeviceToHost));
   return Z2_OK;
@@ -1296,7 +1515,7 @@ extern "C" int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right
 
 // ---------------------------------------------------------------- kernel timing hooks
 extern "C" int z2_profile_kernel(z2_ctx *c, int32_t kind) {
-  if (!c || kind < 0 || kind > Z2_K_OBS) return set_err(Z2_EINVAL, "bad kernel class %d", kind);
+  if (!c || kind < 0 || kind > Z2_K_OBS_TAU) return set_err(Z2_EINVAL, "bad kernel class %d", kind);
   HIPC(hipStreamSynchronize(c->stream));
   c->prof_kind = kind;
   c->prof_used = 0;

@@ -12,6 +12,7 @@ import numpy as np
 from ._lib import Z2ChainDesc, Z2Params, Z2_PREC_FP64, check, lib
 
 OBS_NAMES = ("flux_avg", "chi_flux", "pair_sc_q0", "cdw_pi", "cdw_shift", "cdw_ratio", "density")
+TAU_NAMES = ("spin", "pair_s", "pair_d", "charge_s", "charge_d", "single", "curxx")
 
 
 def _pd(a):
@@ -72,8 +73,30 @@ class Engine:
     def pre(self):
         check(lib().z2_pre(self._ptr))
 
-    def sweep(self, is_beta: bool = True):
-        check(lib().z2_sweep(self._ptr, int(is_beta)))
+    def sweep(self, is_beta: bool = True, toggle: bool = False):
+        """LocalSweep%sweep; toggle adds the time-displaced measurement (needs enable_tau)."""
+        if toggle:
+            check(lib().z2_sweep_tau(self._ptr, int(is_beta), 1))
+        else:
+            check(lib().z2_sweep(self._ptr, int(is_beta)))
+
+    def enable_tau(self):
+        check(lib().z2_enable_tau(self._ptr))
+
+    def dyn_measure(self):
+        """Dyn%reset + Dyn%sweep_R from the current state (accumulators zeroed, not normalised)."""
+        check(lib().z2_dyn_measure(self._ptr))
+
+    def obs_tau(self, chain: int) -> np.ndarray:
+        """[7, Ltrot, N]: spin, pair_s, pair_d, charge_s, charge_d, single, curxx."""
+        o = np.zeros(7 * self.T * self.N)
+        check(lib().z2_get_obs_tau(self._ptr, chain, _pd(o)))
+        return o.reshape(7, self.T, self.N)
+
+    def dyn_drift(self, chain: int):
+        a, b = np.zeros(3), np.zeros(3)
+        check(lib().z2_get_dyn_drift(self._ptr, chain, _pd(a), _pd(b)))
+        return a, b
 
     def sweep_shadow(self, ref: "Engine", is_beta: bool = True) -> np.ndarray:
         """One sweep in lockstep with an FP64 reference engine, replaying its
