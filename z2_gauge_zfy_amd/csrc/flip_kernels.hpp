@@ -38,20 +38,34 @@ struct FlipArgs {
 };
 
 template <typename R>
-__device__ __forceinline__ R flip_read(const R *G, int i, int j, int N, int mu, const ChainConst &c) {
+__device__ __forceinline__ R flip_read(const R *G, int i, int j, int N, int mu, double ep, double em) {
   R x = G[i + (size_t)j * N];
   if (mu) {
-    x = x * (R)c.ep;
-    x = x * (R)c.em;
+    x = x * (R)ep;
+    x = x * (R)em;
   }
   return x;
 }
+
+// LDS bytes of k_flip for N sites and delay depth kd (the host sizes the launch with it)
+__host__ __device__ inline size_t flip_lds_bytes(int N, int kd) {
+  const size_t N2 = 2 * (size_t)N;
+  size_t b = sizeof(double) * (2 * (size_t)N + 3 * N2 + 4 * (size_t)N * kd + 16);
+  b += sizeof(short) * 8 * N2;  // bond sites (2) and plaquette partner bonds (6) per bond
+  b += 2 * N2;                  // current field, sum of the neighbouring slices' fields
+  return (b + 15) & ~(size_t)15;
+}
+
+#define FLIP_CB 8     // columns of a flush tile
+#define FLIP_RR 4     // rows per lane of a flush tile (64 apart)
 
 template <typename R>
 __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   const int N = a.lat.N, N2 = a.lat.N2;
   const int chain = blockIdx.x;
-  const ChainConst c = a.cc[chain];
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const ChainConst *cp = a.cc + chain;
+  const double ep = cp->ep, em = cp->em;
   R *G = a.G + (size_t)chain * N * N;
   int8_t *sigc = a.sig + (size_t)chain * a.T * N2;
   int8_t *sig_t = sigc + (size_t)(a.nt - 1) * N2;
@@ -61,123 +75,263 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   R *Vb = a.Vbuf + chain * a.upd_stride;
 
   extern __shared__ unsigned char smem_raw[];
-  double *dg = reinterpret_cast<double *>(smem_raw);  // N   : G_ii
-  double *o12 = dg + N;                               // 2N  : G_p1p2 per bond
-  double *o21 = o12 + N2;                             // 2N  : G_p2p1 per bond
-  double *rnd = o21 + N2;                             // 2N  : random number per proposal index
-  double *PT = rnd + N2;                              // kdelay x 4N : pending T0,T1,V0,V1
-  int *sg = reinterpret_cast<int *>(PT + (size_t)4 * N * a.kdelay);  // 2N : current slice field
-  __shared__ int s_kmin;
-  __shared__ double s_pi[4], s_d[4];
-  __shared__ int s_p[2];
-  __shared__ int s_nacc, s_tot;
+  double *dgb = reinterpret_cast<double *>(smem_raw);  // 2 x N : G_ii (double-buffered per round)
+  double *o12 = dgb + 2 * N;                           // 2N    : G_p1p2 per bond
+  double *o21 = o12 + N2;                              // 2N    : G_p2p1 per bond
+  double *rnd = o21 + N2;                              // 2N    : random number per proposal index
+  double *PT = rnd + N2;                               // kdelay x 4N : pending T0,T1,V0,V1
+  double *bos = PT + (size_t)4 * N * a.kdelay;         // 9 (16) : boson ratio table
+  short *bp = reinterpret_cast<short *>(bos + 16);     // 2N x 2 : bond sites
+  short *bbs = bp + 2 * N2;                            // 2N x 6 : plaquette partner bonds
+  int8_t *sg = reinterpret_cast<int8_t *>(bbs + 6 * N2);  // 2N : current slice field
+  int8_t *sgs = sg + N2;                                  // 2N : sig(nt-1) + sig(nt+1)
+  __shared__ int s_kmin[2];
   __shared__ uint32_t s_dec[64];  // decision bits of this slice (2N <= 2048)
+
+  // delta for s_old = +1 / -1 in registers (no dynamic indexing of the constants)
+  const double Dp0 = cp->delta[0][0], Dp1 = cp->delta[0][1], Dp2 = cp->delta[0][2], Dp3 = cp->delta[0][3];
+  const double Dm0 = cp->delta[1][0], Dm1 = cp->delta[1][1], Dm2 = cp->delta[1][2], Dm3 = cp->delta[1][3];
 
   int mu = a.mu_on_read;
   int mu_done = 0;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) dg[i] = (double)flip_read<R>(G, i, i, N, mu, c);
+  for (int i = tid; i < N; i += nth) dgb[i] = (double)flip_read<R>(G, i, i, N, mu, ep, em);
   const uint32_t seed0 = (uint32_t)a.seed[chain] & 0x7fffffffu;
-  for (int b = threadIdx.x; b < N2; b += blockDim.x) {
-    int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
-    o12[b] = (double)flip_read<R>(G, p1, p2, N, mu, c);
-    o21[b] = (double)flip_read<R>(G, p2, p1, N, mu, c);
-    sg[b] = (int)sig_t[b];
+  for (int b = tid; b < N2; b += nth) {
+    const int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
+    bp[2 * b] = (short)p1;
+    bp[2 * b + 1] = (short)p2;
+    for (int q = 0; q < 6; ++q) bbs[6 * b + q] = (short)a.lat.bond_bonds[6 * b + q];
+    o12[b] = (double)flip_read<R>(G, p1, p2, N, mu, ep, em);
+    o21[b] = (double)flip_read<R>(G, p2, p1, N, mu, ep, em);
+    sg[b] = sig_t[b];
+    sgs[b] = (int8_t)((int)sig_p[b] + (int)sig_n[b]);
     // proposal index k = b: draw number k+1 of this slice
     uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[b + 1];
     rnd[b] = (double)(uint32_t)(x & 0x7fffffffull) / 2147483648.0;
   }
-  if (threadIdx.x == 0) {
-    s_nacc = 0;
-    s_tot = 0;
+  if (tid < 9) bos[tid] = cp->boson[tid];
+  for (int q = tid; q < 64; q += nth) s_dec[q] = 0u;
+  if (tid == 0) {
+    s_kmin[0] = 2 * N2 + 2;
+    s_kmin[1] = 2 * N2 + 2;
   }
-  for (int q = threadIdx.x; q < 64; q += blockDim.x) s_dec[q] = 0u;
   const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
   __syncthreads();
 
-  int start = 0;
-  while (start < N2) {
-    if (threadIdx.x == 0) s_kmin = N2;
-    __syncthreads();
-    for (int k = start + threadIdx.x; k < N2; k += blockDim.x) {
+  // Rounds: (A) flush if kdelay updates are pending, fold the last accepted
+  // update into the cached 2x2 blocks and evaluate the remaining proposals;
+  // (B) build the rank-2 update of the first accepted one.  The values a
+  // proposal sees are bit-identical to those of the serial loop.
+  int start = 0, na = 0, last = -1, cur = 0, round = 0, tot = 0;
+  while (true) {
+    const int slot = round & 1;
+    if (na >= a.kdelay) {
+      // flush the pending updates into G in acceptance order (and mu, once).
+      // A lane owns FLIP_RR rows (64 apart) x FLIP_CB columns of G in registers,
+      // so every T value read from LDS serves FLIP_CB updates and every V value
+      // FLIP_RR updates; each element still gets the updates one by one, in order.
+      const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, nw = nth >> 6;
+      const int rpb = 64 * FLIP_RR;
+      const int nrb = (N + rpb - 1) / rpb;
+      const int ncb = (N + FLIP_CB - 1) / FLIP_CB;
+      for (int task = wave; task < nrb * ncb; task += nw) {
+        const int rb = task % nrb, cb = (task / nrb) * FLIP_CB;
+        int rr[FLIP_RR];
+        bool rv[FLIP_RR];
+#pragma unroll
+        for (int q = 0; q < FLIP_RR; ++q) {
+          const int r = rb * rpb + q * 64 + lane;
+          rv[q] = r < N;
+          rr[q] = rv[q] ? r : 0;
+        }
+        const int nc = min(FLIP_CB, N - cb);
+        R x[FLIP_RR][FLIP_CB];
+#pragma unroll
+        for (int j = 0; j < FLIP_CB; ++j)
+#pragma unroll
+          for (int q = 0; q < FLIP_RR; ++q) x[q][j] = j < nc ? G[rr[q] + (size_t)(cb + j) * N] : (R)0;
+        if (mu) {
+#pragma unroll
+          for (int j = 0; j < FLIP_CB; ++j)
+#pragma unroll
+            for (int q = 0; q < FLIP_RR; ++q) {
+              x[q][j] = x[q][j] * (R)ep;
+              x[q][j] = x[q][j] * (R)em;
+            }
+        }
+        for (int k = 0; k < na; ++k) {
+          const double *P = PT + (size_t)4 * N * k;
+          R t0[FLIP_RR], t1[FLIP_RR];
+#pragma unroll
+          for (int q = 0; q < FLIP_RR; ++q) {
+            t0[q] = (R)P[rr[q]];
+            t1[q] = (R)P[N + rr[q]];
+          }
+#pragma unroll
+          for (int j = 0; j < FLIP_CB; ++j) {
+            const int cj = j < nc ? cb + j : cb;
+            const R v0 = (R)P[2 * N + cj], v1 = (R)P[3 * N + cj];
+#pragma unroll
+            for (int q = 0; q < FLIP_RR; ++q) {
+              R pp = t0[q] * v0;
+              R qq = t1[q] * v1;
+              R dd = pp + qq;
+              x[q][j] = x[q][j] - dd;
+            }
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < FLIP_CB; ++j)
+#pragma unroll
+          for (int q = 0; q < FLIP_RR; ++q)
+            if (j < nc && rv[q]) G[rr[q] + (size_t)(cb + j) * N] = x[q][j];
+      }
+      if (mu) mu_done = 1;
+      mu = 0;
+      na = 0;
+    }
+    const double *dgc = dgb + (size_t)cur * N;
+    double *dgn = dgb + (size_t)(cur ^ 1) * N;
+    const double *TA = last >= 0 ? PT + (size_t)4 * N * last : nullptr;
+    const double *VA = last >= 0 ? TA + 2 * N : nullptr;
+    if (last >= 0)
+      for (int i = tid; i < N; i += nth) {
+        double d = TA[i] * VA[i] + TA[N + i] * VA[N + i];
+        dgn[i] = dgc[i] - d;
+      }
+    bool found = false;
+    for (int k = tid; k < N2; k += nth) {
+      if (k < start) continue;
       const int b = a.descending ? (N2 - 1 - k) : k;
-      const int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
+      const int p1 = bp[2 * b], p2 = bp[2 * b + 1];
+      double g12 = o12[b], g21 = o21[b], g11 = dgc[p1], g22 = dgc[p2];
+      if (last >= 0) {
+        double d12 = TA[p1] * VA[p2] + TA[N + p1] * VA[N + p2];
+        double d21 = TA[p2] * VA[p1] + TA[N + p2] * VA[N + p1];
+        g12 = g12 - d12;
+        g21 = g21 - d21;
+        o12[b] = g12;
+        o21[b] = g21;
+        double e1 = TA[p1] * VA[p1] + TA[N + p1] * VA[N + p1];
+        double e2 = TA[p2] * VA[p2] + TA[N + p2] * VA[N + p2];
+        g11 = g11 - e1;
+        g22 = g22 - e2;
+      }
+      if (found) continue;
       const int s_old = sg[b];
-      const double *D = c.delta[s_old > 0 ? 0 : 1];
-      const double A11 = 1.0 - dg[p1], A21 = 0.0 - o21[b];
-      const double A12 = 0.0 - o12[b], A22 = 1.0 - dg[p2];
-      const double P11 = 1.0 + (D[0] * A11 + D[2] * A21);
-      const double P21 = 0.0 + (D[1] * A11 + D[3] * A21);
-      const double P12 = 0.0 + (D[0] * A12 + D[2] * A22);
-      const double P22 = 1.0 + (D[1] * A12 + D[3] * A22);
+      const double D0 = s_old > 0 ? Dp0 : Dm0, D1 = s_old > 0 ? Dp1 : Dm1;
+      const double D2 = s_old > 0 ? Dp2 : Dm2, D3 = s_old > 0 ? Dp3 : Dm3;
+      const double A11 = 1.0 - g11, A21 = 0.0 - g21;
+      const double A12 = 0.0 - g12, A22 = 1.0 - g22;
+      const double P11 = 1.0 + (D0 * A11 + D2 * A21);
+      const double P21 = 0.0 + (D1 * A11 + D3 * A21);
+      const double P12 = 0.0 + (D0 * A12 + D2 * A22);
+      const double P22 = 1.0 + (D1 * A12 + D3 * A22);
       const double det = P11 * P22 - P12 * P21;
       // boson ratio: plaquette sums (src/fields.f90:66-101) and temporal term (:104-117)
-      const int *bb = a.lat.bond_bonds + 6 * b;
+      const short *bb = bbs + 6 * b;
       const int s_new = -s_old;
-      int tn1 = s_new * sg[bb[0]] * sg[bb[1]] * sg[bb[2]];
-      int tn2 = s_new * sg[bb[3]] * sg[bb[4]] * sg[bb[5]];
+      int tn1 = s_new * (int)sg[bb[0]] * (int)sg[bb[1]] * (int)sg[bb[2]];
+      int tn2 = s_new * (int)sg[bb[3]] * (int)sg[bb[4]] * (int)sg[bb[5]];
       int dS = 2 * (tn1 + tn2);  // tmp_new - tmp_old with tmp_old = -tmp_new
-      int nb = (int)sig_p[b] + (int)sig_n[b];
-      int dT = 2 * s_new * nb;
-      const double rb = c.boson[(dS / 4 + 1) * 3 + (dT / 4 + 1)];
+      int dT = 2 * s_new * (int)sgs[b];
+      const double rb = bos[(dS / 4 + 1) * 3 + (dT / 4 + 1)];
       const double ratio = (det * det) * rb;
       const bool acc = frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rnd[k];
       if (acc) {
-        atomicMin(&s_kmin, k);
-        break;  // later k of this thread are larger
+        atomicMin(&s_kmin[slot], 2 * k + (s_old > 0 ? 1 : 0));
+        found = true;
       }
     }
+    if (tid == 0) s_kmin[slot ^ 1] = 2 * N2 + 2;  // read by all threads before the last barrier
     __syncthreads();
-    const int kmin = s_kmin;
-    if (kmin >= N2) break;
+    if (last >= 0) cur ^= 1;
+    const int packed = s_kmin[slot];
+    if (packed >= 2 * N2) break;
+    const int kmin = packed >> 1, s_old = (packed & 1) ? 1 : -1;
     const int b = a.descending ? (N2 - 1 - kmin) : kmin;
-    const int na = s_nacc;
-    if (threadIdx.x == 0) {
-      const int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
-      const int s_old = sg[b];
-      const double *D = c.delta[s_old > 0 ? 0 : 1];
-      const double A11 = 1.0 - dg[p1], A21 = 0.0 - o21[b];
-      const double A12 = 0.0 - o12[b], A22 = 1.0 - dg[p2];
-      const double P11 = 1.0 + (D[0] * A11 + D[2] * A21);
-      const double P21 = 0.0 + (D[1] * A11 + D[3] * A21);
-      const double P12 = 0.0 + (D[0] * A12 + D[2] * A22);
-      const double P22 = 1.0 + (D[1] * A12 + D[3] * A22);
-      const double det = P11 * P22 - P12 * P21;
-      s_pi[0] = P22 / det;     // Pinv(1,1)
-      s_pi[1] = (-P21) / det;  // Pinv(2,1)
-      s_pi[2] = (-P12) / det;  // Pinv(1,2)
-      s_pi[3] = P11 / det;     // Pinv(2,2)
-      for (int q = 0; q < 4; ++q) s_d[q] = D[q];
-      s_p[0] = p1;
-      s_p[1] = p2;
-      sg[b] = -s_old;
-      s_dec[kmin >> 5] |= 1u << (kmin & 31);
-      s_tot = s_tot + 1;
+    const int p1 = bp[2 * b], p2 = bp[2 * b + 1];
+    // this thread's first G loads of the update, issued before the 2x2 inverse
+    R g1 = (R)0, g2 = (R)0;
+    if (tid < 2 * N) {
+      if (tid < N) {
+        g1 = G[tid + (size_t)p1 * N];
+        g2 = G[tid + (size_t)p2 * N];
+      } else {
+        g1 = G[p1 + (size_t)(tid - N) * N];
+        g2 = G[p2 + (size_t)(tid - N) * N];
+      }
     }
-    __syncthreads();
-    double *TA = PT + (size_t)4 * N * na;  // slot of this accept: T0, T1, V0, V1
+    const double *dgr = dgb + (size_t)cur * N;
+    const double D11 = s_old > 0 ? Dp0 : Dm0, D21 = s_old > 0 ? Dp1 : Dm1;
+    const double D12 = s_old > 0 ? Dp2 : Dm2, D22 = s_old > 0 ? Dp3 : Dm3;
+    double Pi11, Pi21, Pi12, Pi22;
     {
-      const int p1 = s_p[0], p2 = s_p[1];
-      const double Pi11 = s_pi[0], Pi21 = s_pi[1], Pi12 = s_pi[2], Pi22 = s_pi[3];
-      const double D11 = s_d[0], D21 = s_d[1], D12 = s_d[2], D22 = s_d[3];
-      for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        // current G(i,p1), G(i,p2), G(p1,i), G(p2,i): pending accepts applied in order
-        double c1 = (double)flip_read<R>(G, i, p1, N, mu, c);
-        double c2 = (double)flip_read<R>(G, i, p2, N, mu, c);
-        double r1 = (double)flip_read<R>(G, p1, i, N, mu, c);
-        double r2 = (double)flip_read<R>(G, p2, i, N, mu, c);
+      const double A11 = 1.0 - dgr[p1], A21 = 0.0 - o21[b];
+      const double A12 = 0.0 - o12[b], A22 = 1.0 - dgr[p2];
+      const double P11 = 1.0 + (D11 * A11 + D12 * A21);
+      const double P21 = 0.0 + (D21 * A11 + D22 * A21);
+      const double P12 = 0.0 + (D11 * A12 + D12 * A22);
+      const double P22 = 1.0 + (D21 * A12 + D22 * A22);
+      const double det = P11 * P22 - P12 * P21;
+      Pi11 = P22 / det;
+      Pi21 = (-P21) / det;
+      Pi12 = (-P12) / det;
+      Pi22 = P11 / det;
+    }
+    if (tid == 0) {
+      sg[b] = (int8_t)(-s_old);
+      s_dec[kmin >> 5] |= 1u << (kmin & 31);
+      tot = tot + 1;
+    }
+    double *TN = PT + (size_t)4 * N * na;  // slot of this accept: T0, T1, V0, V1
+    for (int q = tid; q < 2 * N; q += nth) {
+      if (q < N) {
+        // column part: current G(i,p1), G(i,p2) with pending accepts applied in order
+        const int i = q;
+        R x1 = q == tid ? g1 : G[i + (size_t)p1 * N];
+        R x2 = q == tid ? g2 : G[i + (size_t)p2 * N];
+        if (mu) {
+          x1 = x1 * (R)ep;
+          x1 = x1 * (R)em;
+          x2 = x2 * (R)ep;
+          x2 = x2 * (R)em;
+        }
+        double c1 = (double)x1;
+        double c2 = (double)x2;
         for (int k = 0; k < na; ++k) {
           const double *P = PT + (size_t)4 * N * k;
           double ti0 = P[i], ti1 = P[N + i];
-          double tp10 = P[p1], tp11 = P[N + p1], tp20 = P[p2], tp21 = P[N + p2];
-          double vi0 = P[2 * N + i], vi1 = P[3 * N + i];
           double vp10 = P[2 * N + p1], vp11 = P[3 * N + p1], vp20 = P[2 * N + p2], vp21 = P[3 * N + p2];
           c1 = c1 - (ti0 * vp10 + ti1 * vp11);
           c2 = c2 - (ti0 * vp20 + ti1 * vp21);
-          r1 = r1 - (tp10 * vi0 + tp11 * vi1);
-          r2 = r2 - (tp20 * vi0 + tp21 * vi1);
         }
         double t0 = c1 * Pi11 + c2 * Pi21;
         double t1 = c1 * Pi12 + c2 * Pi22;
+        TN[i] = t0;
+        TN[N + i] = t1;
+        Tb[(2 * na) * N + i] = (R)t0;
+        Tb[(2 * na + 1) * N + i] = (R)t1;
+      } else {
+        // row part: current G(p1,i), G(p2,i)
+        const int i = q - N;
+        R x1 = q == tid ? g1 : G[p1 + (size_t)i * N];
+        R x2 = q == tid ? g2 : G[p2 + (size_t)i * N];
+        if (mu) {
+          x1 = x1 * (R)ep;
+          x1 = x1 * (R)em;
+          x2 = x2 * (R)ep;
+          x2 = x2 * (R)em;
+        }
+        double r1 = (double)x1;
+        double r2 = (double)x2;
+        for (int k = 0; k < na; ++k) {
+          const double *P = PT + (size_t)4 * N * k;
+          double tp10 = P[p1], tp11 = P[N + p1], tp20 = P[p2], tp21 = P[N + p2];
+          double vi0 = P[2 * N + i], vi1 = P[3 * N + i];
+          r1 = r1 - (tp10 * vi0 + tp11 * vi1);
+          r2 = r2 - (tp20 * vi0 + tp21 * vi1);
+        }
         double v0 = -(D11 * r1) - D12 * r2;
         double v1 = -(D21 * r1) - D22 * r2;
         if (i == p1) {
@@ -188,71 +342,27 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
           v0 = v0 + D12;
           v1 = v1 + D22;
         }
-        TA[i] = t0;
-        TA[N + i] = t1;
-        TA[2 * N + i] = v0;
-        TA[3 * N + i] = v1;
-        Tb[(2 * na) * N + i] = (R)t0;
-        Tb[(2 * na + 1) * N + i] = (R)t1;
+        TN[2 * N + i] = v0;
+        TN[3 * N + i] = v1;
         Vb[(2 * na) * N + i] = (R)v0;
         Vb[(2 * na + 1) * N + i] = (R)v1;
       }
     }
     __syncthreads();
-    // fold the update into the cached 2x2 blocks
-    const double *VA = TA + 2 * N;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-      double d = TA[i] * VA[i] + TA[N + i] * VA[N + i];
-      dg[i] = dg[i] - d;
-    }
-    for (int bb2 = threadIdx.x; bb2 < N2; bb2 += blockDim.x) {
-      int q1 = a.lat.bond_p[2 * bb2], q2 = a.lat.bond_p[2 * bb2 + 1];
-      double d12 = TA[q1] * VA[q2] + TA[N + q1] * VA[N + q2];
-      double d21 = TA[q2] * VA[q1] + TA[N + q2] * VA[N + q1];
-      o12[bb2] = o12[bb2] - d12;
-      o21[bb2] = o21[bb2] - d21;
-    }
-    if (threadIdx.x == 0) s_nacc = na + 1;
-    __syncthreads();
-    if (na + 1 >= a.kdelay) {
-      // flush the pending updates into G in acceptance order (and mu, once)
-      const int np = na + 1;
-      for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
-        const int col = idx / N, r = idx - col * N;
-        R x = G[idx];
-        if (mu) {
-          x = x * (R)c.ep;
-          x = x * (R)c.em;
-        }
-        for (int k = 0; k < np; ++k) {
-          const double *P = PT + (size_t)4 * N * k;
-          R t0 = (R)P[r], t1 = (R)P[N + r];
-          R v0 = (R)P[2 * N + col], v1 = (R)P[3 * N + col];
-          R pp = t0 * v0;
-          R qq = t1 * v1;
-          R dd = pp + qq;
-          x = x - dd;
-        }
-        G[idx] = x;
-      }
-      if (mu) mu_done = 1;
-      mu = 0;
-      __syncthreads();
-      if (threadIdx.x == 0) s_nacc = 0;
-      __syncthreads();
-    }
+    last = na;
+    na = na + 1;
     start = kmin + 1;
+    round = round + 1;
   }
-  __syncthreads();
-  for (int b = threadIdx.x; b < N2; b += blockDim.x) sig_t[b] = (int8_t)sg[b];
+  for (int b = tid; b < N2; b += nth) sig_t[b] = sg[b];
   if (a.record)
-    for (int q = threadIdx.x; q < a.dwords; q += blockDim.x) a.record[(size_t)chain * a.dwords + q] = s_dec[q];
-  if (threadIdx.x == 0) {
-    a.nacc[chain] = s_nacc;
+    for (int q = tid; q < a.dwords; q += nth) a.record[(size_t)chain * a.dwords + q] = s_dec[q];
+  if (tid == 0) {
+    a.nacc[chain] = na;
     a.mu_done[chain] = mu_done;
     uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[N2];
     a.seed[chain] = (int32_t)(uint32_t)(x & 0x7fffffffull);
-    a.acc_count[2 * chain] += (unsigned long long)s_tot;
+    a.acc_count[2 * chain] += (unsigned long long)tot;
     a.acc_count[2 * chain + 1] += (unsigned long long)N2;
   }
 }

@@ -354,11 +354,12 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
       if (hipGetDevice(&dev) == hipSuccess &&
           hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && lds > 0)
         c->lds_max = (size_t)lds;
-      // flip kernel LDS: (N + 6N) doubles + 2N ints + 4N doubles per pending update
-      size_t base = sizeof(double) * (size_t)(N + 3 * N2) + sizeof(int) * N2 + 1024;
-      long cap = c->lds_max > base ? (long)((c->lds_max - base) / (sizeof(double) * 4 * (size_t)N)) : 0;
+      // flip kernel LDS: cached blocks and tables plus 4N doubles per pending update
+      // (512 bytes kept for its static shared variables)
+      int cap = 0;
+      while (cap < N2 && flip_lds_bytes(N, cap + 1) + 512 <= c->lds_max) ++cap;
       if (cap < 1) { rc = set_err(Z2_EINVAL, "N too large for the flip kernel's LDS"); break; }
-      c->kdelay_max = (int)(cap < N2 ? cap : N2);
+      c->kdelay_max = cap;
       c->kdelay = c->kdelay_max < 16 ? c->kdelay_max : 16;
     }
     if ((rc = dalloc(c, &c->rec, B * c->dwords))) break;
@@ -701,7 +702,7 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
   f.record = c->rec;
   f.dwords = c->dwords;
   f.lat = c->lat;
-  size_t sh = sizeof(double) * ((size_t)c->N + 3 * c->N2 + (size_t)4 * c->N * c->kdelay) + sizeof(int) * c->N2;
+  size_t sh = flip_lds_bytes(c->N, c->kdelay);
   static_assert(FLIP_THREADS <= 512, "k_flip is compiled for at most 512 threads per block");
   if (sh > 65536) HIPC(hipFuncSetAttribute((const void *)k_flip<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   CHK(prof_begin(c, Z2_K_FLIP));
