@@ -123,10 +123,18 @@ int z2_get_obs_equal(z2_ctx *ctx, int32_t chain, double out[7]);
 /* cumulative local-flip counts: accepted, proposed */
 int z2_get_counters(z2_ctx *ctx, int32_t chain, uint64_t out[2]);
 /* delay depth k of the local-update kernel: accepted rank-2 updates are kept
- * pending (in LDS) and applied to G in blocks of k (default min(16, kmax(N)),
- * kmax set by the LDS size; FP64 results do not depend on k, the updates are
- * applied in acceptance order) */
+ * pending (in LDS) and applied to G as one rank-2k matrix-core product per k
+ * accepts (default min(16, kmax(N)), kmax set by the LDS size); the decisions
+ * do not depend on k, G only through rounding */
 int z2_set_delay(z2_ctx *ctx, int32_t k);
+/* QR used by the stabilization (no counterpart in the reference, which calls
+ * ZGEQP3): Z2_QR_BLOCKED (default) orders the columns once by decreasing norm
+ * and runs a blocked Householder QR with block-reflector updates on the matrix
+ * cores; Z2_QR_PIVOTED keeps ZGEQP3's column pivoting by downdated norms. Both
+ * give A P = Q R with the same output convention; G agrees to rounding. */
+#define Z2_QR_BLOCKED 0
+#define Z2_QR_PIVOTED 1
+int z2_set_qr_mode(z2_ctx *ctx, int32_t mode);
 /* HIP stream of the context (hipStream_t as void*) for timing by the caller */
 int z2_get_stream(z2_ctx *ctx, void **stream);
 

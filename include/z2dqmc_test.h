@@ -14,6 +14,9 @@ int z2_test_gemm(int32_t n, const double *A, int32_t ta, const double *B, int32_
                  int32_t triA, int32_t triB, double *C);
 /* QDRP_decompose (src/stabilization.f90:42-68) in place on A; Q (optional) = explicit Q */
 int z2_test_qrp(int32_t n, double *A, double *D, int32_t *ipvt, double *tau, double *Q);
+/* the blocked QR with columns pre-ordered by norm (same outputs as z2_test_qrp,
+ * ipvt the pre-ordering) and its explicit Q */
+int z2_test_qr_blocked(int32_t n, double *A, double *D, int32_t *ipvt, double *tau, double *Q);
 /* X <- R^{-1} X (right = 0, ZTRSM L,U,N,N) or X R^{-T} (right = 1, ZTRSM R,U,C,N) */
 int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right);
 #ifdef __cplusplus

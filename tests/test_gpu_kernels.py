@@ -56,6 +56,34 @@ def test_qrp_matches_oracle(L, n):
                                atol=1e-11 * np.abs(A0).max())
 
 
+@pytest.mark.parametrize("n", [16, 100, 144, 256, 400, 512])
+def test_qr_blocked(L, n):
+    """Blocked QR with columns pre-ordered by norm: A P = Q diag(D) R, Q orthogonal,
+    R the Householder R of the ordered matrix (numpy's LAPACK QR, same sign rule)."""
+    from z2_gauge_zfy_amd._lib import check
+    rng = np.random.default_rng(100 + n)
+    A0 = rng.standard_normal((n, n)) @ np.diag(np.logspace(0, 10, n))[:, rng.permutation(n)]
+    A = np.asfortranarray(A0.copy())
+    D = np.zeros(n)
+    ip = np.zeros(n, np.int32)
+    tau = np.zeros(n)
+    Q = np.zeros((n, n), order="F")
+    check(L.z2_test_qr_blocked(n, _pd(A), _pd(D), ip.ctypes.data_as(C.POINTER(C.c_int32)), _pd(tau),
+                               _pd(Q)))
+    norms = np.linalg.norm(A0, axis=0)
+    assert sorted(ip - 1) == list(range(n))
+    assert np.all(np.diff(norms[ip - 1]) <= 0)  # decreasing column norms
+    Ap = A0[:, ip - 1]
+    R = np.triu(A) * D[:, None]
+    np.testing.assert_allclose(np.abs(np.diag(np.triu(A))), 1.0, rtol=1e-14)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(n), atol=1e-13)
+    np.testing.assert_allclose(Q @ R, Ap, rtol=1e-11, atol=1e-11 * np.abs(A0).max())
+    Rn = np.linalg.qr(Ap, mode="r")
+    scale = np.abs(Rn).max(axis=1, keepdims=True)
+    assert np.abs(np.abs(R) - np.abs(Rn)).max() < 1e-10 * scale.max()
+    np.testing.assert_allclose(D, np.abs(np.diag(Rn)), rtol=1e-9)
+
+
 @pytest.mark.parametrize("n", [16, 130])
 @pytest.mark.parametrize("right", [0, 1])
 def test_trsm(L, n, right):

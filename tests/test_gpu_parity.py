@@ -77,10 +77,14 @@ def test_parity_batched_uneven_couplings_and_mu():
     _run_parity(6, 20, 2.0, 5, specs, nsweeps=3)
 
 
-@pytest.mark.parametrize("kdelay", [1, 3, 32])
-def test_slice_kernels_bit_exact(kdelay):
-    """wraps + flips + delayed updates of single slices, no stabilization:
-    bit-identical to the oracle (FP64, no FMA contraction) for any delay depth."""
+@pytest.mark.parametrize("kdelay", [1, 3, 32, 72])
+def test_slice_kernels(kdelay):
+    """wraps + flips + delayed updates of single slices, no stabilization.  The
+    decisions (field, seed) are identical to the oracle's for every delay depth.
+    Without in-kernel flushes (kdelay = 2N) the pending updates are applied by
+    the kinetic kernels one by one in acceptance order and G is bit-identical to
+    the oracle's (FP64, no FMA contraction); the in-kernel flush applies them as
+    one matrix-core product, which rounds differently: G within 1e-13."""
     import z2_gauge_zfy_amd as z
     spec = z.ChainSpec(h=0.4, mu=0.2, Beta=2.0, seed=777)
     L, T = 6, 20
@@ -92,16 +96,23 @@ def test_slice_kernels_bit_exact(kdelay):
     G = np.asfortranarray(rng.standard_normal((L * L, L * L)) * 0.05 + 0.5 * np.eye(L * L))
     e.set_green(0, G)
     c.set_green(G)
+
+    def same_G(where):
+        if kdelay >= 2 * L * L:
+            assert np.array_equal(e.green(0), c.green()), where
+        else:
+            _close_G(e.green(0), c.green(), tol=1e-13)
+
     for nt in (7, 6, 5):
         e.slice_L(nt)
         c.slice_L(nt)
         assert np.array_equal(e.sigma(0), c.sigma())
-        assert np.array_equal(e.green(0), c.green()), f"slice_L {nt}"
+        same_G(f"slice_L {nt}")
     for nt in (5, 6, 7):
         e.slice_R(nt)
         c.slice_R(nt)
         assert np.array_equal(e.sigma(0), c.sigma())
-        assert np.array_equal(e.green(0), c.green()), f"slice_R {nt}"
+        same_G(f"slice_R {nt}")
     assert e.seed(0) == c.seed
 
 

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(PKG, "libz2dqmc.so")
 
 Z2_PREC_FP64 = 0
 Z2_PREC_MIXED = 1
+Z2_QR_BLOCKED = 0
+Z2_QR_PIVOTED = 1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
@@ -74,6 +76,7 @@ _SIGS = {
     "z2_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "z2_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "z2_set_delay": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_set_qr_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     # kernel test hooks (include/z2dqmc_test.h)
@@ -83,6 +86,9 @@ _SIGS = {
     "z2_test_qrp": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                               C.POINTER(C.c_int32), C.POINTER(C.c_double),
                               C.POINTER(C.c_double)]),
+    "z2_test_qr_blocked": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "z2_test_trsm": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.c_int32]),
 }

@@ -9,9 +9,10 @@
 // and folded into the cached 2x2 blocks of every bond, and evaluation resumes
 // after it.  Proposals before the first accepted one were evaluated against the
 // exact state the serial loop would have seen, so the accept/reject history is
-// the serial one.  The recorded updates are applied to G in acceptance order:
-// by this kernel whenever `kdelay` of them are pending (the delay depth), and
-// the remaining ones by the next kinetic kernel (flush prologue).
+// the serial one.  The recorded updates are applied to G by this kernel as one
+// rank-2k product on the matrix cores whenever k = `kdelay` of them are pending
+// (the delay depth), and the remaining ones by the next kinetic kernel (flush
+// prologue, in acceptance order).
 #pragma once
 #include "z2_common.hpp"
 
@@ -56,8 +57,27 @@ __host__ __device__ inline size_t flip_lds_bytes(int N, int kd) {
   return (b + 15) & ~(size_t)15;
 }
 
-#define FLIP_CB 8     // columns of a flush tile
-#define FLIP_RR 4     // rows per lane of a flush tile (64 apart)
+// 16x16x4 matrix-core product in the precision of the propagated G:
+// lane l supplies A[l%16][l/16] and B[l/16][l%16]; item q holds
+// D[mfma_row(q, l/16)][l%16]
+template <typename R> struct MfmaAcc;
+template <> struct MfmaAcc<double> { typedef double type __attribute__((ext_vector_type(4))); };
+template <> struct MfmaAcc<float> { typedef float type __attribute__((ext_vector_type(4))); };
+// output row of item q in lane group lk = lane/16 (the two precisions differ)
+template <typename R> __device__ __forceinline__ int mfma_row(int q, int lk);
+template <> __device__ __forceinline__ int mfma_row<double>(int q, int lk) { return 4 * q + lk; }
+template <> __device__ __forceinline__ int mfma_row<float>(int q, int lk) { return 4 * lk + q; }
+#define FLIP_BPW 4  // flush: column blocks per wavefront task
+template <typename R>
+__device__ __forceinline__ typename MfmaAcc<R>::type mfma_16x16x4(R a, R b, typename MfmaAcc<R>::type c);
+template <>
+__device__ __forceinline__ MfmaAcc<double>::type mfma_16x16x4<double>(double a, double b, MfmaAcc<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ MfmaAcc<float>::type mfma_16x16x4<float>(float a, float b, MfmaAcc<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
 
 template <typename R>
 __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
@@ -126,65 +146,48 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   while (true) {
     const int slot = round & 1;
     if (na >= a.kdelay) {
-      // flush the pending updates into G in acceptance order (and mu, once).
-      // A lane owns FLIP_RR rows (64 apart) x FLIP_CB columns of G in registers,
-      // so every T value read from LDS serves FLIP_CB updates and every V value
-      // FLIP_RR updates; each element still gets the updates one by one, in order.
+      // flush the pending updates into G as one rank-2na product on the matrix
+      // cores (and mu, once): G(r, c) -= sum_k T_k(r) V_k(c).  Output blocks are
+      // computed transposed (O = -V T^T) so that the 16 lanes of an output row
+      // cover 16 consecutive rows of G; a wavefront keeps FLIP_BPW column blocks
+      // of one row block in flight (they share the T operand).
       const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, nw = nth >> 6;
-      const int rpb = 64 * FLIP_RR;
-      const int nrb = (N + rpb - 1) / rpb;
-      const int ncb = (N + FLIP_CB - 1) / FLIP_CB;
-      for (int task = wave; task < nrb * ncb; task += nw) {
-        const int rb = task % nrb, cb = (task / nrb) * FLIP_CB;
-        int rr[FLIP_RR];
-        bool rv[FLIP_RR];
+      const int li = lane & 15, lk = lane >> 4;
+      const int nbk = (N + 15) >> 4, ncg = (nbk + FLIP_BPW - 1) / FLIP_BPW, K = 2 * na;
+      for (int task = wave; task < nbk * ncg; task += nw) {
+        const int r = 16 * (task % nbk) + li;
+        const int cb0 = FLIP_BPW * 16 * (task / nbk);
+        typename MfmaAcc<R>::type acc[FLIP_BPW];
 #pragma unroll
-        for (int q = 0; q < FLIP_RR; ++q) {
-          const int r = rb * rpb + q * 64 + lane;
-          rv[q] = r < N;
-          rr[q] = rv[q] ? r : 0;
-        }
-        const int nc = min(FLIP_CB, N - cb);
-        R x[FLIP_RR][FLIP_CB];
+        for (int u = 0; u < FLIP_BPW; ++u)
 #pragma unroll
-        for (int j = 0; j < FLIP_CB; ++j)
-#pragma unroll
-          for (int q = 0; q < FLIP_RR; ++q) x[q][j] = j < nc ? G[rr[q] + (size_t)(cb + j) * N] : (R)0;
-        if (mu) {
-#pragma unroll
-          for (int j = 0; j < FLIP_CB; ++j)
-#pragma unroll
-            for (int q = 0; q < FLIP_RR; ++q) {
-              x[q][j] = x[q][j] * (R)ep;
-              x[q][j] = x[q][j] * (R)em;
+          for (int q = 0; q < 4; ++q) {
+            const int col = cb0 + 16 * u + mfma_row<R>(q, lk);
+            R x = (r < N && col < N) ? G[r + (size_t)col * N] : (R)0;
+            if (mu) {
+              x = x * (R)ep;
+              x = x * (R)em;
             }
-        }
-        for (int k = 0; k < na; ++k) {
-          const double *P = PT + (size_t)4 * N * k;
-          R t0[FLIP_RR], t1[FLIP_RR];
-#pragma unroll
-          for (int q = 0; q < FLIP_RR; ++q) {
-            t0[q] = (R)P[rr[q]];
-            t1[q] = (R)P[N + rr[q]];
+            acc[u][q] = x;
           }
+        for (int kk = 0; kk < K; kk += 4) {
+          const int kq = kk + lk;
+          const double *P = PT + (size_t)4 * N * (kq >> 1) + (size_t)(kq & 1) * N;
+          const R bv = (kq < K && r < N) ? (R)P[r] : (R)0;  // T_kq(r)
 #pragma unroll
-          for (int j = 0; j < FLIP_CB; ++j) {
-            const int cj = j < nc ? cb + j : cb;
-            const R v0 = (R)P[2 * N + cj], v1 = (R)P[3 * N + cj];
-#pragma unroll
-            for (int q = 0; q < FLIP_RR; ++q) {
-              R pp = t0[q] * v0;
-              R qq = t1[q] * v1;
-              R dd = pp + qq;
-              x[q][j] = x[q][j] - dd;
-            }
+          for (int u = 0; u < FLIP_BPW; ++u) {
+            const int c = cb0 + 16 * u + li;
+            const R av = (kq < K && c < N) ? (R)(-P[2 * N + c]) : (R)0;  // -V_kq(c)
+            acc[u] = mfma_16x16x4<R>(av, bv, acc[u]);
           }
         }
 #pragma unroll
-        for (int j = 0; j < FLIP_CB; ++j)
+        for (int u = 0; u < FLIP_BPW; ++u)
 #pragma unroll
-          for (int q = 0; q < FLIP_RR; ++q)
-            if (j < nc && rv[q]) G[rr[q] + (size_t)(cb + j) * N] = x[q][j];
+          for (int q = 0; q < 4; ++q) {
+            const int col = cb0 + 16 * u + mfma_row<R>(q, lk);
+            if (r < N && col < N) G[r + (size_t)col * N] = acc[u][q];
+          }
       }
       if (mu) mu_done = 1;
       mu = 0;

@@ -16,6 +16,7 @@
 #include "kin_kernels.hpp"
 #include "linalg_kernels.hpp"
 #include "obs_kernels.hpp"
+#include "qr_blocked.hpp"
 #include "tau_kernels.hpp"
 
 static thread_local std::string g_err;
@@ -86,6 +87,8 @@ struct z2_ctx {
   double *M = nullptr, *Q = nullptr, *X = nullptr, *Y = nullptr, *Gn = nullptr;
   double *tau = nullptr, *Dtmp = nullptr;
   int *ipvt = nullptr;
+  double *qrw = nullptr, *qrw2 = nullptr;  // workspaces of the blocked QR (N x N, 2N x 2N)
+  int qr_mode = Z2_QR_BLOCKED;
   // mixed precision: propagated G, UUR, UUL and the delayed updates in FP32
   bool mixed = false;
   float *Gf = nullptr, *UURf = nullptr, *UULf = nullptr, *Tbf = nullptr, *Vbf = nullptr;
@@ -348,6 +351,7 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if ((rc = dalloc(c, &c->tau, B * N))) break;
     if ((rc = dalloc(c, &c->Dtmp, B * N))) break;
     if ((rc = dalloc(c, &c->ipvt, B * N))) break;
+    if ((rc = dalloc(c, &c->qrw, B * nn))) break;
     c->dwords = (N2 + 31) / 32;
     {
       int dev = 0, lds = 0;
@@ -428,6 +432,12 @@ extern "C" int z2_set_delay(z2_ctx *c, int32_t k) {
   if (!c || k < 1 || k > c->kdelay_max)
     return set_err(Z2_EINVAL, "delay depth must be in [1, %d] for N = %d", c ? c->kdelay_max : 0, c ? c->N : 0);
   c->kdelay = k;
+  return Z2_OK;
+}
+
+extern "C" int z2_set_qr_mode(z2_ctx *c, int32_t mode) {
+  if (!c || (mode != Z2_QR_BLOCKED && mode != Z2_QR_PIVOTED)) return set_err(Z2_EINVAL, "bad QR mode");
+  c->qr_mode = mode;
   return Z2_OK;
 }
 
@@ -840,7 +850,55 @@ static int launch_orgqr(int N, dim3 grid, hipStream_t st, const double *A, const
 
 // QR of the chains' n x n matrices A (column-pivoted unless nopiv): R with rows
 // scaled to unit diagonal magnitude in the upper triangle, D = |diag R|
+// panel width and trailing tile of the blocked QR that fit the LDS for order n (0: none)
+static bool qrb_config(size_t lds_max, int n, int *nb, int *tc) {
+  static const int cand[3][2] = {{32, 32}, {32, 16}, {16, 16}};
+  for (auto &k : cand)
+    if (qrb_lds_bytes(n, k[0], k[1]) <= lds_max) {
+      *nb = k[0];
+      *tc = k[1];
+      return true;
+    }
+  return false;
+}
+
+// one launch of the blocked QR (orgqr = 0) or of its explicit Q (orgqr = 1)
+static int launch_qrb(int orgqr, int nb, int nmat, size_t sh, hipStream_t st, const QrbArgs &q) {
+  const void *fn = orgqr ? (nb == 32 ? (const void *)k_orgqr_blk<32> : (const void *)k_orgqr_blk<16>)
+                         : (nb == 32 ? (const void *)k_qr_blk<32> : (const void *)k_qr_blk<16>);
+  if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  if (orgqr) {
+    if (nb == 32) hipLaunchKernelGGL(k_orgqr_blk<32>, dim3(nmat), dim3(512), sh, st, q);
+    else hipLaunchKernelGGL(k_orgqr_blk<16>, dim3(nmat), dim3(512), sh, st, q);
+  } else {
+    if (nb == 32) hipLaunchKernelGGL(k_qr_blk<32>, dim3(nmat), dim3(512), sh, st, q);
+    else hipLaunchKernelGGL(k_qr_blk<16>, dim3(nmat), dim3(512), sh, st, q);
+  }
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+static double *qr_workspace(z2_ctx *c, int n) { return n == c->N ? c->qrw : (n == 2 * c->N ? c->qrw2 : nullptr); }
+
 static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n, int nopiv) {
+  int nb = 0, tc = 0;
+  double *ws = qr_workspace(c, n);
+  if (c->qr_mode == Z2_QR_BLOCKED && ws && qrb_config(c->lds_max, n, &nb, &tc)) {
+    QrbArgs q;
+    memset(&q, 0, sizeof(q));
+    q.A = A;
+    q.W = ws;
+    q.tau = tau;
+    q.ipvt = ipvt;
+    q.D = D;
+    q.N = n;
+    q.tc = tc;
+    q.nopiv = nopiv;
+    CHK(prof_begin(c, Z2_K_QRP));
+    CHK(launch_qrb(0, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
+    CHK(prof_end(c, Z2_K_QRP));
+    return Z2_OK;
+  }
   QrpArgs q;
   q.A = A;
   q.tau = tau;
@@ -858,6 +916,20 @@ static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n,
 static int qrp(z2_ctx *c, double *A, double *D) { return qrp_n(c, A, D, c->tau, c->ipvt, c->N, 0); }
 
 static int orgqr_n(z2_ctx *c, const double *A, const double *tau, double *Qo, int n) {
+  int nb = 0, tc = 0;
+  if (c->qr_mode == Z2_QR_BLOCKED && qr_workspace(c, n) && qrb_config(c->lds_max, n, &nb, &tc)) {
+    QrbArgs q;
+    memset(&q, 0, sizeof(q));
+    q.Ain = A;
+    q.tin = tau;
+    q.Q = Qo;
+    q.N = n;
+    q.tc = tc;
+    CHK(prof_begin(c, Z2_K_ORGQR));
+    CHK(launch_qrb(1, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
+    CHK(prof_end(c, Z2_K_ORGQR));
+    return Z2_OK;
+  }
   CHK(prof_begin(c, Z2_K_ORGQR));
   CHK(launch_orgqr(n, dim3(c->nch), c->stream, A, tau, Qo));
   CHK(prof_end(c, Z2_K_ORGQR));
@@ -1117,6 +1189,7 @@ extern "C" int z2_enable_tau(z2_ctx *c) {
   for (double **m : {&c->bigA, &c->bigQ, &c->bigZ, &c->bigR}) CHK(dalloc(c, m, B * n2n2));
   for (double **d : {&c->dDUR, &c->dDUL}) CHK(dalloc(c, d, B * c->N));
   CHK(dalloc(c, &c->tau2, B * 2 * c->N));
+  CHK(dalloc(c, &c->qrw2, B * n2n2));
   CHK(dalloc(c, &c->D2, B * 2 * c->N));
   CHK(dalloc(c, &c->ipvt2, B * 2 * c->N));
   CHK(dalloc(c, &c->otau, B * TAU_NOBS * c->T * c->N));
@@ -1487,6 +1560,37 @@ extern "C" int z2_test_qrp(int32_t n, double *A, double *D, int32_t *ipvt, doubl
   size_t sh = sizeof(double) * (2 * n + 32) + sizeof(int) * 32;
   CHK(launch_qrp(n, dim3(1), sh, 0, q));
   if (Q) CHK(launch_orgqr(n, dim3(1), 0, (const double *)dA, (const double *)dt, dQ));
+  HIPC(hipDeviceSynchronize());
+  HIPC(hipMemcpy(A, dA, nn * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(D, dD, n * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(tau, dt, n * 8, hipMemcpyDeviceToHost));
+  HIPC(hipMemcpy(ipvt, dp, n * 4, hipMemcpyDeviceToHost));
+  if (Q) HIPC(hipMemcpy(Q, dQ, nn * 8, hipMemcpyDeviceToHost));
+  return Z2_OK;
+}
+
+extern "C" int z2_test_qr_blocked(int32_t n, double *A, double *D, int32_t *ipvt, double *tau, double *Q) {
+  if (n < 1 || !A || !D || !ipvt || !tau) return set_err(Z2_EINVAL, "bad argument");
+  int dev = 0, lds = 65536;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || lds <= 0)
+    lds = 65536;
+  int nb = 0, tc = 0;
+  if (!qrb_config((size_t)lds, n, &nb, &tc)) return set_err(Z2_EINVAL, "order %d too large for the blocked QR", n);
+  TmpBufs t;
+  size_t nn = (size_t)n * n;
+  double *dA = t.get<double>(nn), *dW = t.get<double>(nn), *dD = t.get<double>(n), *dt = t.get<double>(n);
+  double *dQ = t.get<double>(nn);
+  int *dp = t.get<int>(n);
+  if (!dA || !dW || !dD || !dt || !dp || !dQ) return set_err(Z2_ENOMEM, "alloc");
+  HIPC(hipMemcpy(dA, A, nn * 8, hipMemcpyHostToDevice));
+  QrbArgs q;
+  memset(&q, 0, sizeof(q));
+  q.A = dA; q.W = dW; q.tau = dt; q.ipvt = dp; q.D = dD; q.N = n; q.tc = tc; q.nopiv = 0;
+  q.Ain = dA; q.tin = dt; q.Q = dQ;
+  const size_t sh = qrb_lds_bytes(n, nb, tc);
+  CHK(launch_qrb(0, nb, 1, sh, 0, q));
+  if (Q) CHK(launch_qrb(1, nb, 1, sh, 0, q));
   HIPC(hipDeviceSynchronize());
   HIPC(hipMemcpy(A, dA, nn * 8, hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(D, dD, n * 8, hipMemcpyDeviceToHost));

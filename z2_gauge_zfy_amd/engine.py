@@ -161,6 +161,10 @@ class Engine:
     def set_delay(self, k: int):
         check(lib().z2_set_delay(self._ptr, int(k)))
 
+    def set_qr_mode(self, mode: int):
+        """Z2_QR_BLOCKED (columns pre-ordered by norm, blocked) or Z2_QR_PIVOTED (ZGEQP3 pivoting)."""
+        check(lib().z2_set_qr_mode(self._ptr, int(mode)))
+
     def profile_kernel(self, name: str):
         from ._lib import KERNEL_IDS
         check(lib().z2_profile_kernel(self._ptr, KERNEL_IDS[name] if name else 0))
