@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 METRIC = "DQMC sweeps/s (all chains) at L=16,β=20: mixed-prec vs FP64; max|ΔG|"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 
 CONFIGS = {
     # name: (L, Ltrot, beta, Nwrap, n_chains, h values, mu)
@@ -42,9 +43,12 @@ def chain_specs(name):
                         seed=12345 + 2 * 7919 * k, absolute=1) for k in range(nch)]
 
 
-# algorithmic work of one launch of a kernel class, per chain (N sites)
+# algorithmic work of one launch of a kernel class, per chain (N sites); the
+# flip kernel's work is 4 N^2 flops per accepted rank-2 update (counted separately)
 def kernel_work(kind, N):
-    if kind == "qrp":      # Householder QR with column pivoting: 4/3 N^3 flops
+    if kind == "flip":
+        return ("mfma", 0.0)
+    if kind == "qrp":      # Householder QR: 4/3 N^3 flops
         return ("mfma", 4.0 / 3.0 * N ** 3)
     if kind == "orgqr":    # explicit Q: 4/3 N^3 flops
         return ("mfma", 4.0 / 3.0 * N ** 3)
@@ -56,9 +60,34 @@ def kernel_work(kind, N):
         return ("hbm", 2.0 * 8 * N * N)
     if kind == "obs":      # read G twice (G_ij, G_ji)
         return ("hbm", 2.0 * 8 * N * N)
-    if kind == "flip":     # 2x2 blocks of all 2N bonds + 2 rows/cols per accept (lower bound)
-        return ("hbm", 8.0 * 5 * N)
     raise KeyError(kind)
+
+
+def roofline(kind, N, nch, kms, klaunch, accepted, precision, traffic=None):
+    """achieved rate of one kernel class from its summed HIP-event time"""
+    bound, work = kernel_work(kind, N)
+    avg_s = kms / 1e3 / max(1, klaunch)
+    if kind == "flip":
+        per_launch = 4.0 * N * N * accepted / max(1, klaunch)
+    else:
+        per_launch = work * nch
+    if bound == "hbm":
+        achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
+    else:
+        peak = FP32_PEAK_TFLOPS if (precision == "mixed" and kind == "flip") else FP64_PEAK_TFLOPS
+        achieved, unit = per_launch / avg_s / 1e12, "TFLOP/s"
+    return {"kernel": kind, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
+            "launches": klaunch, "work_per_launch": per_launch}
+
+
+def measured_traffic(kind, cfg, precision):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any"""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kind}_{cfg}_{precision}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
 
 
 def _cpu_worker(args):
@@ -89,8 +118,18 @@ def cpu_baseline(cfg, nslices):
                       f"max over processes {max(times):.2f} s"}
 
 
-def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay):
-    """Time `steps` sweeps of all chains after `warmup` sweeps; returns a dict."""
+PROGRESS = False
+
+
+def progress(msg):
+    if PROGRESS:
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay, extra=()):
+    """Time `steps` sweeps of all chains after `warmup` sweeps (the kernel class
+    `roofline_kernel` bracketed by HIP events inside the timed region); then one
+    more sweep per class in `extra` for their own event timing."""
     import z2_gauge_zfy_amd as z
     L, T, beta, w, nch, hs, mu = CONFIGS[cfg]
     N = L * L
@@ -99,25 +138,42 @@ def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay):
         eng.set_delay(delay)
     eng.init_fields()
     eng.pre()
-    for _ in range(warmup):
+    for i in range(warmup):
         eng.sweep(True)
+        eng.sync()
+        progress(f"{precision} warm-up sweep {i + 1}/{warmup} done")
     eng.sync()
     acc0 = sum(eng.counters(k)[0] for k in range(nch))
     prop0 = sum(eng.counters(k)[1] for k in range(nch))
     eng.profile_kernel(roofline_kernel)
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for i in range(steps):
         eng.sweep(True)
+        if PROGRESS:
+            eng.sync()
+            progress(f"{precision} timed sweep {i + 1}/{steps} done")
     eng.sync()
     dt = time.perf_counter() - t0
     kms, klaunch = eng.profile_read()
     eng.profile_kernel("")
     acc1 = sum(eng.counters(k)[0] for k in range(nch))
     prop1 = sum(eng.counters(k)[1] for k in range(nch))
+    others = {}
+    for kind in extra:
+        a0 = sum(eng.counters(k)[0] for k in range(nch))
+        eng.profile_kernel(kind)
+        t1 = time.perf_counter()
+        eng.sweep(True)
+        eng.sync()
+        t_sw = time.perf_counter() - t1
+        ms, nl = eng.profile_read()
+        eng.profile_kernel("")
+        a1 = sum(eng.counters(k)[0] for k in range(nch))
+        others[kind] = (ms, nl, a1 - a0, ms / 1e3 / t_sw)
     eng.close()
     return dict(dt=dt, sweeps_per_s=nch * steps / dt, flips_per_s=(prop1 - prop0) / dt,
                 acc_per_s=(acc1 - acc0) / dt, acceptance=(acc1 - acc0) / max(1, prop1 - prop0),
-                kms=kms, klaunch=klaunch)
+                kms=kms, klaunch=klaunch, accepted=acc1 - acc0, others=others)
 
 
 def shadow_max_dG(cfg, sweeps, warmup, delay):
@@ -151,12 +207,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c4", choices=sorted(CONFIGS))
     ap.add_argument("--precision", default="both", choices=("both", "mixed", "fp64"))
-    ap.add_argument("--roofline-kernel", default="qrp")
+    ap.add_argument("--roofline-kernel", default="flip")
+    ap.add_argument("--extra-kernels", default="qrp,orgqr,kin_rows,kin_cols",
+                    help="kernel classes timed in one extra sweep each (roofline_other)")
     ap.add_argument("--delay", type=int, default=0, help="delay depth k (0: engine default)")
     ap.add_argument("--cpu-slices", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-shadow", action="store_true")
+    ap.add_argument("--progress", action="store_true", help="one stderr line per sweep")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="first run this many launches of a plain 64 MiB copy kernel (k_copy), "
+                         "the reference for scaling rocprofv3 byte counters")
     a = ap.parse_args()
+    global PROGRESS
+    PROGRESS = a.progress
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}, WORLD_SIZE={world}); "
@@ -167,21 +231,16 @@ def main():
     import z2_gauge_zfy_amd as z
     L, T, beta, w, nch, hs, mu = CONFIGS[a.config]
     N = L * L
+    if a.copy_calibration:
+        z._lib.check(z.lib().z2_test_copy(8 << 20, a.copy_calibration))
+    extra = [k for k in a.extra_kernels.split(",") if k and k != "none"]
     res = {}
     if a.precision in ("both", "fp64"):
-        res["fp64"] = run_mode(a.config, z.Z2_PREC_FP64, a.steps, a.warmup, a.roofline_kernel, a.delay)
+        res["fp64"] = run_mode(a.config, z.Z2_PREC_FP64, a.steps, a.warmup, a.roofline_kernel, a.delay, extra)
     if a.precision in ("both", "mixed"):
-        res["mixed"] = run_mode(a.config, z.Z2_PREC_MIXED, a.steps, a.warmup, a.roofline_kernel, a.delay)
+        res["mixed"] = run_mode(a.config, z.Z2_PREC_MIXED, a.steps, a.warmup, a.roofline_kernel, a.delay, extra)
     head = res["mixed"] if "mixed" in res else res["fp64"]
     head_name = "mixed" if "mixed" in res else "fp64"
-
-    bound, work = kernel_work(a.roofline_kernel, N)
-    avg_s = head["kms"] / 1e3 / max(1, head["klaunch"])
-    per_launch = work * nch
-    if bound == "hbm":
-        achieved, peak, unit = per_launch / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
-    else:
-        achieved, peak, unit = per_launch / avg_s / 1e12, FP64_PEAK_TFLOPS, "TFLOP/s"
     out = {
         "metric": METRIC,
         "value": head["sweeps_per_s"],
@@ -207,10 +266,16 @@ def main():
         out[f"{k}_sweeps_per_s"] = r["sweeps_per_s"]
     if "mixed" in res and "fp64" in res:
         out["mixed_over_fp64"] = res["mixed"]["sweeps_per_s"] / res["fp64"]["sweeps_per_s"]
-    out["roofline"] = {"kernel": a.roofline_kernel, "bound": bound, "achieved": achieved, "peak": peak,
-                       "unit": unit, "frac": achieved / peak, "traffic": None,
-                       "avg_launch_us": avg_s * 1e6, "launches": head["klaunch"],
-                       "kernel_share_of_step": (head["kms"] / 1e3) / head["dt"]}
+    rf = roofline(a.roofline_kernel, N, nch, head["kms"], head["klaunch"], head["accepted"], head_name,
+                  measured_traffic(a.roofline_kernel, a.config, head_name))
+    rf["kernel_share_of_step"] = (head["kms"] / 1e3) / head["dt"]
+    out["roofline"] = rf
+    other = []
+    for kind, (ms, nl, acc, share) in head["others"].items():
+        r = roofline(kind, N, nch, ms, nl, acc, head_name, measured_traffic(kind, a.config, head_name))
+        r["kernel_share_of_sweep"] = share
+        other.append(r)
+    out["roofline_other"] = other
     if a.precision == "both" and not a.no_shadow:
         dg, dobs = shadow_max_dG(a.config, 1, a.warmup, a.delay)
         out["max_dG"] = dg

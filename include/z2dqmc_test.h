@@ -17,6 +17,9 @@ int z2_test_qrp(int32_t n, double *A, double *D, int32_t *ipvt, double *tau, dou
 /* the blocked QR with columns pre-ordered by norm (same outputs as z2_test_qrp,
  * ipvt the pre-ordering) and its explicit Q */
 int z2_test_qr_blocked(int32_t n, double *A, double *D, int32_t *ipvt, double *tau, double *Q);
+/* reps launches of a plain copy kernel (k_copy) of n doubles: reads and writes
+ * 8 n bytes each, the reference traffic for scaling hardware byte counters */
+int z2_test_copy(int64_t n, int32_t reps);
 /* X <- R^{-1} X (right = 0, ZTRSM L,U,N,N) or X R^{-T} (right = 1, ZTRSM R,U,C,N) */
 int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right);
 #ifdef __cplusplus

@@ -89,6 +89,7 @@ _SIGS = {
     "z2_test_qr_blocked": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                      C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]),
+    "z2_test_copy": (C.c_int, [C.c_int64, C.c_int32]),
     "z2_test_trsm": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                C.c_int32]),
 }

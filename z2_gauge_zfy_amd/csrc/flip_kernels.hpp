@@ -79,7 +79,19 @@ __device__ __forceinline__ MfmaAcc<float>::type mfma_16x16x4<float>(float a, flo
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-template <typename R>
+// minimum over the 64 lanes of a wavefront (all lanes active), in every lane
+__device__ __forceinline__ int wave_min_int(int v) {
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false));   // quad_perm(1,0,3,2)
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false));   // quad_perm(2,3,0,1)
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = min(v, __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, false));  // row_mirror
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// MPT: proposals per thread (2N <= 512 MPT); a thread keeps the lattice data,
+// random number and current 2x2 block of its proposals in registers.
+template <typename R, int MPT>
 __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   const int N = a.lat.N, N2 = a.lat.N2;
   const int chain = blockIdx.x;
@@ -137,6 +149,28 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   }
   const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
   __syncthreads();
+  // per-thread proposals k = tid + m * nth: bond, sites, partner bonds, field,
+  // random number, 2x2 block (G_p1p1, G_p2p2, G_p1p2, G_p2p1), all in registers
+  int rb_[MPT], rp1[MPT], rp2[MPT], rbb[MPT][6], rso[MPT], rss[MPT];
+  double rrn[MPT], r11[MPT], r22[MPT], r12[MPT], r21[MPT];
+#pragma unroll
+  for (int m = 0; m < MPT; ++m) {
+    const int k = tid + m * nth;
+    const int kc = k < N2 ? k : 0;
+    const int b = a.descending ? (N2 - 1 - kc) : kc;
+    rb_[m] = b;
+    rp1[m] = bp[2 * b];
+    rp2[m] = bp[2 * b + 1];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) rbb[m][q] = bbs[6 * b + q];
+    rso[m] = sg[b];
+    rss[m] = sgs[b];
+    rrn[m] = rnd[kc];
+    r11[m] = dgb[rp1[m]];
+    r22[m] = dgb[rp2[m]];
+    r12[m] = o12[b];
+    r21[m] = o21[b];
+  }
 
   // Rounds: (A) flush if kdelay updates are pending, fold the last accepted
   // update into the cached 2x2 blocks and evaluate the remaining proposals;
@@ -153,13 +187,14 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       // of one row block in flight (they share the T operand).
       const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, nw = nth >> 6;
       const int li = lane & 15, lk = lane >> 4;
-      const int nbk = (N + 15) >> 4, ncg = (nbk + FLIP_BPW - 1) / FLIP_BPW, K = 2 * na;
+      constexpr int BPW = FLIP_BPW;
+      const int nbk = (N + 15) >> 4, ncg = (nbk + BPW - 1) / BPW, K = 2 * na;
       for (int task = wave; task < nbk * ncg; task += nw) {
         const int r = 16 * (task % nbk) + li;
-        const int cb0 = FLIP_BPW * 16 * (task / nbk);
-        typename MfmaAcc<R>::type acc[FLIP_BPW];
+        const int cb0 = BPW * 16 * (task / nbk);
+        typename MfmaAcc<R>::type acc[BPW];
 #pragma unroll
-        for (int u = 0; u < FLIP_BPW; ++u)
+        for (int u = 0; u < BPW; ++u)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int col = cb0 + 16 * u + mfma_row<R>(q, lk);
@@ -175,14 +210,14 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
           const double *P = PT + (size_t)4 * N * (kq >> 1) + (size_t)(kq & 1) * N;
           const R bv = (kq < K && r < N) ? (R)P[r] : (R)0;  // T_kq(r)
 #pragma unroll
-          for (int u = 0; u < FLIP_BPW; ++u) {
+          for (int u = 0; u < BPW; ++u) {
             const int c = cb0 + 16 * u + li;
             const R av = (kq < K && c < N) ? (R)(-P[2 * N + c]) : (R)0;  // -V_kq(c)
             acc[u] = mfma_16x16x4<R>(av, bv, acc[u]);
           }
         }
 #pragma unroll
-        for (int u = 0; u < FLIP_BPW; ++u)
+        for (int u = 0; u < BPW; ++u)
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int col = cb0 + 16 * u + mfma_row<R>(q, lk);
@@ -202,50 +237,48 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
         double d = TA[i] * VA[i] + TA[N + i] * VA[N + i];
         dgn[i] = dgc[i] - d;
       }
-    bool found = false;
-    for (int k = tid; k < N2; k += nth) {
-      if (k < start) continue;
-      const int b = a.descending ? (N2 - 1 - k) : k;
-      const int p1 = bp[2 * b], p2 = bp[2 * b + 1];
-      double g12 = o12[b], g21 = o21[b], g11 = dgc[p1], g22 = dgc[p2];
+    int cand = 2 * N2 + 2;
+#pragma unroll
+    for (int m = 0; m < MPT; ++m) {
+      const int k = tid + m * nth;
+      if (k >= N2 || k < start) continue;
+      const int p1 = rp1[m], p2 = rp2[m];
       if (last >= 0) {
         double d12 = TA[p1] * VA[p2] + TA[N + p1] * VA[N + p2];
         double d21 = TA[p2] * VA[p1] + TA[N + p2] * VA[N + p1];
-        g12 = g12 - d12;
-        g21 = g21 - d21;
-        o12[b] = g12;
-        o21[b] = g21;
+        r12[m] = r12[m] - d12;
+        r21[m] = r21[m] - d21;
+        o12[rb_[m]] = r12[m];
+        o21[rb_[m]] = r21[m];
         double e1 = TA[p1] * VA[p1] + TA[N + p1] * VA[N + p1];
         double e2 = TA[p2] * VA[p2] + TA[N + p2] * VA[N + p2];
-        g11 = g11 - e1;
-        g22 = g22 - e2;
+        r11[m] = r11[m] - e1;
+        r22[m] = r22[m] - e2;
       }
-      if (found) continue;
-      const int s_old = sg[b];
+      if (cand < 2 * N2) continue;  // an earlier proposal of this thread is accepted
+      const int s_old = rso[m];
       const double D0 = s_old > 0 ? Dp0 : Dm0, D1 = s_old > 0 ? Dp1 : Dm1;
       const double D2 = s_old > 0 ? Dp2 : Dm2, D3 = s_old > 0 ? Dp3 : Dm3;
-      const double A11 = 1.0 - g11, A21 = 0.0 - g21;
-      const double A12 = 0.0 - g12, A22 = 1.0 - g22;
+      const double A11 = 1.0 - r11[m], A21 = 0.0 - r21[m];
+      const double A12 = 0.0 - r12[m], A22 = 1.0 - r22[m];
       const double P11 = 1.0 + (D0 * A11 + D2 * A21);
       const double P21 = 0.0 + (D1 * A11 + D3 * A21);
       const double P12 = 0.0 + (D0 * A12 + D2 * A22);
       const double P22 = 1.0 + (D1 * A12 + D3 * A22);
       const double det = P11 * P22 - P12 * P21;
       // boson ratio: plaquette sums (src/fields.f90:66-101) and temporal term (:104-117)
-      const short *bb = bbs + 6 * b;
       const int s_new = -s_old;
-      int tn1 = s_new * (int)sg[bb[0]] * (int)sg[bb[1]] * (int)sg[bb[2]];
-      int tn2 = s_new * (int)sg[bb[3]] * (int)sg[bb[4]] * (int)sg[bb[5]];
+      int tn1 = s_new * (int)sg[rbb[m][0]] * (int)sg[rbb[m][1]] * (int)sg[rbb[m][2]];
+      int tn2 = s_new * (int)sg[rbb[m][3]] * (int)sg[rbb[m][4]] * (int)sg[rbb[m][5]];
       int dS = 2 * (tn1 + tn2);  // tmp_new - tmp_old with tmp_old = -tmp_new
-      int dT = 2 * s_new * (int)sgs[b];
+      int dT = 2 * s_new * rss[m];
       const double rb = bos[(dS / 4 + 1) * 3 + (dT / 4 + 1)];
       const double ratio = (det * det) * rb;
-      const bool acc = frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rnd[k];
-      if (acc) {
-        atomicMin(&s_kmin[slot], 2 * k + (s_old > 0 ? 1 : 0));
-        found = true;
-      }
+      const bool acc = frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rrn[m];
+      if (acc) cand = 2 * k + (s_old > 0 ? 1 : 0);
     }
+    cand = wave_min_int(cand);
+    if ((tid & 63) == 0 && cand < 2 * N2) atomicMin(&s_kmin[slot], cand);
     if (tid == 0) s_kmin[slot ^ 1] = 2 * N2 + 2;  // read by all threads before the last barrier
     __syncthreads();
     if (last >= 0) cur ^= 1;
@@ -309,12 +342,8 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
           c1 = c1 - (ti0 * vp10 + ti1 * vp11);
           c2 = c2 - (ti0 * vp20 + ti1 * vp21);
         }
-        double t0 = c1 * Pi11 + c2 * Pi21;
-        double t1 = c1 * Pi12 + c2 * Pi22;
-        TN[i] = t0;
-        TN[N + i] = t1;
-        Tb[(2 * na) * N + i] = (R)t0;
-        Tb[(2 * na + 1) * N + i] = (R)t1;
+        TN[i] = c1 * Pi11 + c2 * Pi21;
+        TN[N + i] = c1 * Pi12 + c2 * Pi22;
       } else {
         // row part: current G(p1,i), G(p2,i)
         const int i = q - N;
@@ -347,8 +376,6 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
         }
         TN[2 * N + i] = v0;
         TN[3 * N + i] = v1;
-        Vb[(2 * na) * N + i] = (R)v0;
-        Vb[(2 * na + 1) * N + i] = (R)v1;
       }
     }
     __syncthreads();
@@ -356,6 +383,15 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
     na = na + 1;
     start = kmin + 1;
     round = round + 1;
+  }
+  // the pending updates left for the next kinetic kernel
+  for (int idx = tid; idx < na * N; idx += nth) {
+    const int k = idx / N, i = idx - k * N;
+    const double *P = PT + (size_t)4 * N * k;
+    Tb[(2 * k) * N + i] = (R)P[i];
+    Tb[(2 * k + 1) * N + i] = (R)P[N + i];
+    Vb[(2 * k) * N + i] = (R)P[2 * N + i];
+    Vb[(2 * k + 1) * N + i] = (R)P[3 * N + i];
   }
   for (int b = tid; b < N2; b += nth) sig_t[b] = sg[b];
   if (a.record)

@@ -20,6 +20,7 @@ struct ChainConst {
 // lattice tables shared by all chains of one context (same L)
 struct LatticeDev {
   int N, N2;
+  int Lx, Ly;
   const int *bond_p;       // 2N x 2 sites (0-based): [b*2+k]
   const int *grp;          // 4 groups x (N/2) bonds (0-based), group g at grp + g*(N/2)
   const int *bond_bonds;   // 2N x 6 neighbour bonds (0-based): [b*6+k]

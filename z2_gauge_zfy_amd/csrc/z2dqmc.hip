@@ -290,6 +290,8 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if (hipMemcpy(c->d_int, all.data(), sizeof(int) * all.size(), hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "copy tables"); break; }
     c->lat.N = N;
     c->lat.N2 = N2;
+    c->lat.Lx = Lx;
+    c->lat.Ly = Ly;
     c->lat.bond_p = c->d_int + off_bond;
     c->lat.grp = c->d_int + off_grp;
     c->lat.bond_bonds = c->d_int + off_bb;
@@ -713,10 +715,15 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
   f.dwords = c->dwords;
   f.lat = c->lat;
   size_t sh = flip_lds_bytes(c->N, c->kdelay);
-  static_assert(FLIP_THREADS <= 512, "k_flip is compiled for at most 512 threads per block");
-  if (sh > 65536) HIPC(hipFuncSetAttribute((const void *)k_flip<R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  static_assert(FLIP_THREADS == 512, "k_flip is compiled for 512 threads per block");
+  const int mpt = (c->N2 + FLIP_THREADS - 1) / FLIP_THREADS;
+  const void *fn = mpt == 1 ? (const void *)k_flip<R, 1> : (mpt == 2 ? (const void *)k_flip<R, 2> : (const void *)k_flip<R, 4>);
+  if (mpt > 4) return set_err(Z2_EINVAL, "flip kernel supports 2N <= %d", 4 * FLIP_THREADS);
+  if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   CHK(prof_begin(c, Z2_K_FLIP));
-  hipLaunchKernelGGL((k_flip<R>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
+  if (mpt == 1) hipLaunchKernelGGL((k_flip<R, 1>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
+  else if (mpt == 2) hipLaunchKernelGGL((k_flip<R, 2>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
+  else hipLaunchKernelGGL((k_flip<R, 4>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
   LAUNCH_CHECK();
   CHK(prof_end(c, Z2_K_FLIP));
   return Z2_OK;
@@ -1597,6 +1604,27 @@ extern "C" int z2_test_qr_blocked(int32_t n, double *A, double *D, int32_t *ipvt
   HIPC(hipMemcpy(tau, dt, n * 8, hipMemcpyDeviceToHost));
   HIPC(hipMemcpy(ipvt, dp, n * 4, hipMemcpyDeviceToHost));
   if (Q) HIPC(hipMemcpy(Q, dQ, nn * 8, hipMemcpyDeviceToHost));
+  return Z2_OK;
+}
+
+// plain streaming copy (calibrates hardware byte counters against a known traffic)
+__global__ void __launch_bounds__(256) k_copy(const double *__restrict__ in, double *__restrict__ out, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = in[i];
+}
+
+extern "C" int z2_test_copy(int64_t n, int32_t reps) {
+  if (n < 1 || reps < 1) return set_err(Z2_EINVAL, "bad argument");
+  TmpBufs t;
+  double *a = t.get<double>((size_t)n), *b = t.get<double>((size_t)n);
+  if (!a || !b) return set_err(Z2_ENOMEM, "alloc");
+  HIPC(hipMemset(a, 0, sizeof(double) * (size_t)n));
+  HIPC(hipDeviceSynchronize());
+  for (int r = 0; r < reps; ++r) {
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, (const double *)a, b, (size_t)n);
+    LAUNCH_CHECK();
+  }
+  HIPC(hipDeviceSynchronize());
   return Z2_OK;
 }
 
