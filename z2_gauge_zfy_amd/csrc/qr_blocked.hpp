@@ -38,8 +38,8 @@ typedef double qrb_d4 __attribute__((ext_vector_type(4)));
 // T and W (ld nb+1), column norms, tau/beta of the panel
 __host__ __device__ inline size_t qrb_lds_bytes(int N, int nb, int tc) {
   const size_t ld = (size_t)N + 1, lw = (size_t)nb + 1;
-  size_t d = ld * nb + ld * tc + lw * nb + lw * (tc > nb ? tc : nb) + N + 2 * (size_t)nb;
-  return sizeof(double) * d;
+  size_t d = ld * nb + ld * tc + lw * nb + lw * (tc > nb ? tc : nb) + N + 2 * (size_t)nb + 2 * ld;
+  return sizeof(double) * d + sizeof(int) * (size_t)N;
 }
 
 struct QrbLds {
@@ -50,6 +50,8 @@ struct QrbLds {
   double *W1;   // nb x max(tc, nb): V^T C, and the Gram matrix V^T V while T is built
   double *nrm;  // N
   double *s_tau, *s_beta;  // nb
+  double *vb;   // 2 x ld: reflector of the current panel column (double-buffered)
+  int *perm;    // N: column order
 };
 
 __device__ __forceinline__ QrbLds qrb_carve(unsigned char *raw, int N, int nb, int tc) {
@@ -63,14 +65,24 @@ __device__ __forceinline__ QrbLds qrb_carve(unsigned char *raw, int N, int nb, i
   L.W1 = p; p += (size_t)L.lw * (tc > nb ? tc : nb);
   L.nrm = p; p += N;
   L.s_tau = p; p += nb;
-  L.s_beta = p;
+  L.s_beta = p; p += nb;
+  L.vb = p; p += 2 * (size_t)L.ld;
+  L.perm = reinterpret_cast<int *>(p);
   return L;
 }
 
-// element (r, c) of the unit lower-trapezoidal reflector block held in Pn (m rows, w columns)
+// element (r, c) of the unit lower-trapezoidal reflector block held in Pn (m rows,
+// w columns); the LDS load is unconditional (clamped address) so that no branch
+// separates consecutive loads
 __device__ __forceinline__ double qrb_v(const double *Pn, int ld, int m, int w, int r, int c) {
-  if (r >= m || c >= w || r < c) return 0.0;
-  return r == c ? 1.0 : Pn[(size_t)c * ld + r];
+  const double x = Pn[(size_t)min(c, w - 1) * ld + min(r, m - 1)];
+  return (r >= m || c >= w || r < c) ? 0.0 : (r == c ? 1.0 : x);
+}
+
+// p[i] if ok else 0, with an unconditional load of p[ok ? i : 0]
+__device__ __forceinline__ double qrb_ld(const double *p, size_t i, bool ok) {
+  const double x = p[ok ? i : 0];
+  return ok ? x : 0.0;
 }
 
 template <int CTRL>
@@ -86,77 +98,203 @@ __device__ __forceinline__ double qrb_readlane(double v, int l) {
   return __hiloint2double(hi, lo);
 }
 
-// sum over the 64 lanes of a wavefront, result in every lane (all lanes active)
+// DPP move into the rows selected by ROWMASK, 0 elsewhere
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double qrb_dpp_rows(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWMASK, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWMASK, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// sum over the 64 lanes of a wavefront, result in every lane (all lanes active):
+// row sums by quad permutes and mirrors, then row_bcast:15 / row_bcast:31 carry the
+// row sums up to lane 63, which is read once
 __device__ __forceinline__ double qrb_wave_sum(double v) {
   v += qrb_dpp<0xB1>(v);   // quad_perm(1,0,3,2)
   v += qrb_dpp<0x4E>(v);   // quad_perm(2,3,0,1)
   v += qrb_dpp<0x141>(v);  // row_half_mirror
   v += qrb_dpp<0x140>(v);  // row_mirror: every lane holds its 16-lane row sum
-  return (qrb_readlane(v, 0) + qrb_readlane(v, 16)) + (qrb_readlane(v, 32) + qrb_readlane(v, 48));
+  v += qrb_dpp_rows<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += qrb_dpp_rows<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3: lane 63 has the total
+  return qrb_readlane(v, 63);
 }
 
-// Householder factorization of the panel Pn (m x w) in LDS: reflectors below
-// the diagonal, alpha kept on it (beta in s_beta), tau in s_tau.  Column c is
-// owned by wave c % nw; a wave updates its columns together.
-template <int NB>
-__device__ void qrb_panel(QrbLds &L, int m, int w, int last_row) {
+// Householder factorization of a panel (m x w, w <= NB) in registers: reflectors
+// below the diagonal, alpha kept on it (beta in s_beta), tau in s_tau.  Wavefront w owns columns
+// c = w + 8 q, lane l rows r = l + 64 t (RPL = ceil(N / 64)).  One barrier per
+// column; the reflector reaches the other wavefronts through LDS (double-buffered),
+// and the owner of column k+1 computes its reflector right after applying
+// reflector k to it, while the other wavefronts are still updating (look-ahead).
+// Reads the panel (m x w) from global memory: column c at Wp + col(c) * ldW with
+// col(c) = cperm[c] (or c); leaves the factored panel in L.Pn.
+template <int NB, int RPL>
+__device__ __forceinline__ void qrb_reflector(QrbLds &L, double (&x)[NB / 8][RPL], int k, int m, int last_row,
+                                              int lane) {
+  constexpr int NW = 8, CPW = NB / NW;
+  const int qk = k / NW;
+  double col[RPL];
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    col[t] = x[0][t];
+#pragma unroll
+    for (int q = 1; q < CPW; ++q)
+      if (q == qk) col[t] = x[q][t];
+  }
+  double s = 0.0, ak = 0.0;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = lane + 64 * t;
+    if (r > k && r < m) s += col[t] * col[t];
+    if (t == (k >> 6)) ak = col[t];
+  }
+  s = qrb_wave_sum(s);
+  const double xnorm = sqrt(s), alpha = qrb_readlane(ak, k & 63);
+  double tau, beta, scale;
+  if (xnorm == 0.0 || k == last_row) {
+    tau = 0.0;
+    beta = alpha;
+    scale = 1.0;
+  } else {
+    const double xa = fabs(alpha), w_ = fmax(xa, xnorm), z_ = fmin(xa, xnorm);
+    const double q_ = z_ / w_;
+    const double nrm = w_ * sqrt(1.0 + q_ * q_);
+    beta = -copysign(nrm, alpha);
+    tau = (beta - alpha) / beta;
+    scale = 1.0 / (alpha - beta);
+  }
+  double *vb = L.vb + (size_t)(k & 1) * L.ld;
+#pragma unroll
+  for (int t = 0; t < RPL; ++t) {
+    const int r = lane + 64 * t;
+    if (r > k && r < m) col[t] = col[t] * scale;
+    if (r < m) vb[r] = (r < k) ? 0.0 : (r == k ? 1.0 : col[t]);
+  }
+#pragma unroll
+  for (int q = 0; q < CPW; ++q)
+    if (q == qk)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) x[q][t] = col[t];
+  if (lane == 0) {
+    L.s_tau[k] = tau;
+    L.s_beta[k] = beta;
+  }
+}
+
+template <int NB, int RPL>
+__device__ void qrb_panel_reg(QrbLds &L, const double *Wp, int ldW, const int *cperm, int m, int w,
+                              int last_row) {
   constexpr int NW = 8, CPW = NB / NW;  // 512 threads
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ld = L.ld;
-  for (int k = 0; k < w; ++k) {
-    if (wave == k % NW) {
-      double *ck = L.Pn + (size_t)k * ld;
-      double s = 0.0;
-      for (int r = k + 1 + lane; r < m; r += 64) s += ck[r] * ck[r];
-      s = qrb_wave_sum(s);
-      const double xnorm = sqrt(s), alpha = ck[k];
-      double tau, beta, scale;
-      if (xnorm == 0.0 || k == last_row) {
-        tau = 0.0;
-        beta = alpha;
-        scale = 1.0;
-      } else {
-        const double xa = fabs(alpha), w_ = fmax(xa, xnorm), z_ = fmin(xa, xnorm);
-        const double q = z_ / w_;
-        const double nrm = w_ * sqrt(1.0 + q * q);
-        beta = -copysign(nrm, alpha);
-        tau = (beta - alpha) / beta;
-        scale = 1.0 / (alpha - beta);
-      }
-      for (int r = k + 1 + lane; r < m; r += 64) ck[r] = ck[r] * scale;
-      if (lane == 0) {
-        L.s_tau[k] = tau;
-        L.s_beta[k] = beta;
-      }
+  double x[CPW][RPL];
+#pragma unroll
+  for (int q = 0; q < CPW; ++q) {
+    const int c = wave + NW * q;
+    const double *src = Wp + (size_t)(c < w ? (cperm ? cperm[c] : c) : 0) * ldW;
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = lane + 64 * t;
+      x[q][t] = qrb_ld(src, (size_t)r, c < w && r < m);
     }
-    __syncthreads();
+  }
+  if (wave == 0) qrb_reflector<NB, RPL>(L, x, 0, m, last_row, lane);
+  __syncthreads();
+  for (int k = 0; k < w; ++k) {
     const double tk = L.s_tau[k];
+    const double *vb = L.vb + (size_t)(k & 1) * L.ld;
+    double vr[RPL];
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = lane + 64 * t;
+      vr[t] = qrb_ld(vb, (size_t)r, r < m);
+    }
+    const int k1 = k + 1;
+    const bool own1 = k1 < w && wave == k1 % NW;
+    if (own1) {
+      // column k+1 first, then its reflector (published for the next step)
+      const int q1 = k1 / NW;
+      if (tk != 0.0) {
+        double col[RPL], d = 0.0;
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          col[t] = x[0][t];
+#pragma unroll
+          for (int q = 1; q < CPW; ++q)
+            if (q == q1) col[t] = x[q][t];
+          d += vr[t] * col[t];
+        }
+        d = qrb_wave_sum(d);
+        const double tt = -tk * d;
+#pragma unroll
+        for (int q = 0; q < CPW; ++q)
+          if (q == q1)
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) x[q][t] = col[t] + vr[t] * tt;
+      }
+      qrb_reflector<NB, RPL>(L, x, k1, m, last_row, lane);
+    }
     if (tk != 0.0) {
-      const double *vk = L.Pn + (size_t)k * ld;
       double d[CPW];
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
-        const int c = wave + q * NW;
         d[q] = 0.0;
-        if (c > k && c < w) {
-          const double *cc = L.Pn + (size_t)c * ld;
-          for (int r = k + lane; r < m; r += 64) d[q] += (r == k ? 1.0 : vk[r]) * cc[r];
-        }
+        const int c = wave + NW * q;
+        if (c > k1 || (c == k1 && !own1))
+          if (c < w)
+#pragma unroll
+            for (int t = 0; t < RPL; ++t) d[q] += vr[t] * x[q][t];
       }
 #pragma unroll
       for (int q = 0; q < CPW; ++q) d[q] = qrb_wave_sum(d[q]);
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
-        const int c = wave + q * NW;
-        if (c > k && c < w) {
-          double *cc = L.Pn + (size_t)c * ld;
-          const double t = -tk * d[q];
-          for (int r = k + lane; r < m; r += 64) cc[r] = cc[r] + (r == k ? 1.0 : vk[r]) * t;
+        const int c = wave + NW * q;
+        if ((c > k1 || (c == k1 && !own1)) && c < w) {
+          const double tt = -tk * d[q];
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) x[q][t] = x[q][t] + vr[t] * tt;
         }
       }
     }
+    __syncthreads();
   }
+#pragma unroll
+  for (int q = 0; q < CPW; ++q)
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int c = wave + NW * q, r = lane + 64 * t;
+      if (c < w && r < m) L.Pn[(size_t)c * L.ld + r] = x[q][t];
+    }
   __syncthreads();
+}
+
+// global <-> LDS tile transfers: wavefront w moves columns w, w + 8, ..., lane l rows
+// l + 64 t; all loads of a thread are issued before its LDS stores
+template <int RPL>
+__device__ __forceinline__ void qrb_tile_load(double *dst, int ldd, const double *src, size_t lds, const int *cperm,
+                                              int m, int ncol) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c0 = wave; c0 < ncol; c0 += 32) {
+    double v[4][RPL];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + 8 * u;
+      const double *s = src + (size_t)(c < ncol ? (cperm ? cperm[c] : c) : 0) * lds;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        v[u][t] = qrb_ld(s, (size_t)r, c < ncol && r < m);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int c = c0 + 8 * u;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        if (c < ncol && r < m) dst[(size_t)c * ldd + r] = v[u][t];
+      }
+    }
+  }
 }
 
 // out (w x ncol, ld L.lw) = V^T B, B = the tile Ct (bv = 0) or V itself (bv = 1).
@@ -171,26 +309,24 @@ __device__ void qrb_vt_times(QrbLds &L, int m, int w, int ncol, int bv, double *
     const int bi = blk % nbi, bj = blk / nbi;
     const int col = 16 * bj + li;
     const int kbeg = 16 * bi;  // V is zero above its diagonal
-    qrb_d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    for (int kk = kbeg; kk < m; kk += 8) {
-      const int r0 = kk + lk, r1 = kk + 4 + lk;
-      const double a0 = qrb_v(L.Pn, ld, m, w, r0, 16 * bi + li);
-      const double a1 = qrb_v(L.Pn, ld, m, w, r1, 16 * bi + li);
-      double b0, b1;
-      if (bv) {
-        b0 = qrb_v(L.Pn, ld, m, w, r0, col);
-        b1 = qrb_v(L.Pn, ld, m, w, r1, col);
-      } else {
-        b0 = (r0 < m && col < ncol) ? L.Ct[(size_t)col * ld + r0] : 0.0;
-        b1 = (r1 < m && col < ncol) ? L.Ct[(size_t)col * ld + r1] : 0.0;
+    qrb_d4 acc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[u] = (qrb_d4){0.0, 0.0, 0.0, 0.0};
+    for (int kk = kbeg; kk < m; kk += 16) {
+      double av[4], bw[4];  // four K steps of operands loaded together, four chains
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int r = kk + 4 * u + lk;
+        av[u] = qrb_v(L.Pn, ld, m, w, r, 16 * bi + li);
+        bw[u] = bv ? qrb_v(L.Pn, ld, m, w, r, col) : qrb_ld(L.Ct, (size_t)col * ld + r, r < m && col < ncol);
       }
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bw[u], acc[u], 0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int ri = 16 * bi + 4 * q + lk, cj = 16 * bj + li;
-      out[ri + (size_t)cj * L.lw] = acc0[q] + acc1[q];
+      out[ri + (size_t)cj * L.lw] = (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
     }
   }
 }
@@ -238,23 +374,29 @@ __device__ void qrb_apply_tile(QrbLds &L, int m, int w, int tcn, int trans) {
   const int ld = L.ld, lw = L.lw;
   qrb_vt_times(L, m, w, tcn, 0, L.W1);
   __syncthreads();
-  // W1 <- op(T) W1: entries e = tid + 512 u (u < 2 covers NB x tc <= 1024),
-  // computed into registers, written back after everyone has read W1
+  // W1 <- op(T) W1: entries e = tid + 512 u (u < 2 covers NB x tc <= 1024), into
+  // registers, written back after everyone has read W1.  T is stored with its
+  // zeros (triangle, rows/columns >= w) and W1 has rows up to lmax, so the sums
+  // run unconditionally.
+  const int lmax = 16 * ((w + 15) >> 4);
   double v[2] = {0.0, 0.0};
 #pragma unroll
   for (int u = 0; u < 2; ++u) {
-    const int e = threadIdx.x + 512 * u, i = e % NB, j = e / NB;
-    if (j < tcn && i < w) {
-      if (trans) {
+    const int e = threadIdx.x + 512 * u, i = e % NB, j = min(e / NB, tcn - 1);
+    const double *wj = L.W1 + (size_t)j * lw;
+    double part[4] = {0.0, 0.0, 0.0, 0.0};  // four interleaved partial sums, 8 terms per batch
+    for (int l0 = 0; l0 < lmax; l0 += 8) {     // lmax is a multiple of 16
+      double tv[8], wv[8];
 #pragma unroll
-        for (int l = 0; l < NB; ++l)
-          if (l <= i) v[u] += L.Tm[l + (size_t)i * lw] * L.W1[l + (size_t)j * lw];
-      } else {
-#pragma unroll
-        for (int l = 0; l < NB; ++l)
-          if (l >= i && l < w) v[u] += L.Tm[i + (size_t)l * lw] * L.W1[l + (size_t)j * lw];
+      for (int q = 0; q < 8; ++q) {
+        const int l = l0 + q;
+        tv[q] = trans ? L.Tm[l + (size_t)i * lw] : L.Tm[i + (size_t)l * lw];
+        wv[q] = wj[l];
       }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) part[q & 3] += tv[q] * wv[q];
     }
+    v[u] = (part[0] + part[1]) + (part[2] + part[3]);
   }
   __syncthreads();
 #pragma unroll
@@ -263,36 +405,57 @@ __device__ void qrb_apply_tile(QrbLds &L, int m, int w, int tcn, int trans) {
     if (j < tcn) L.W1[i + (size_t)j * lw] = v[u];
   }
   __syncthreads();
-  // C -= V W1
+  // C -= V W1: a wavefront takes row blocks wave and wave + 8 (then + 16, ...) and
+  // both column blocks, so its accumulators are independent chains sharing operands
   const int nbm = (m + 15) >> 4, nbj = (tcn + 15) >> 4;
-  for (int blk = wave; blk < nbm * nbj; blk += NW) {
-    const int bm = blk % nbm, bj = blk / nbm;
-    const int col = 16 * bj + li;
-    qrb_d4 acc;
+  for (int bm0 = wave; bm0 < nbm; bm0 += NW * 2) {
+    qrb_d4 acc[2][2];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = 16 * bm + 4 * q + lk;
-      acc[q] = (r < m && col < tcn) ? L.Ct[(size_t)col * ld + r] : 0.0;
-    }
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int kk = 0; kk < NB; kk += 4) {
-      if (kk < w) {
-        const int k = kk + lk;
-        const double a = -qrb_v(L.Pn, ld, m, w, 16 * bm + li, k);
-        const double b = (k < w && col < tcn) ? L.W1[k + (size_t)col * lw] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      for (int j = 0; j < 2; ++j) {
+        const int bm = bm0 + NW * i, col = 16 * j + li;
+        const bool ok = bm < nbm && j < nbj && col < tcn;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 16 * bm + 4 * q + lk;
+          acc[i][j][q] = qrb_ld(L.Ct, (size_t)col * ld + r, ok && r < m);
+        }
+      }
+#pragma unroll
+    for (int u = 0; u < NB / 4; ++u) {
+      if (4 * u < w) {
+        const int k = 4 * u + lk;
+        double av[2], bw[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) av[i] = -qrb_v(L.Pn, ld, m, w, 16 * (bm0 + NW * i) + li, k);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) bw[j] = qrb_ld(L.W1, k + (size_t)(16 * j + li) * lw, k < w && 16 * j + li < tcn);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            if (bm0 + NW * i < nbm && j < nbj)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bw[j], acc[i][j], 0, 0, 0);
       }
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int r = 16 * bm + 4 * q + lk;
-      if (r < m && col < tcn) L.Ct[(size_t)col * ld + r] = acc[q];
-    }
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int bm = bm0 + NW * i, col = 16 * j + li;
+        if (bm < nbm && j < nbj && col < tcn)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = 16 * bm + 4 * q + lk;
+            if (r < m) L.Ct[(size_t)col * ld + r] = acc[i][j][q];
+          }
+      }
   }
   __syncthreads();
 }
 
-template <int NB>
+template <int NB, int RPL>
 __global__ void __launch_bounds__(512) k_qr_blk(QrbArgs a) {
   const int N = a.N, tc = a.tc;
   const int chain = blockIdx.x;
@@ -304,14 +467,29 @@ __global__ void __launch_bounds__(512) k_qr_blk(QrbArgs a) {
   double *D = a.D + (size_t)chain * N;
   extern __shared__ unsigned char smem_raw[];
   QrbLds L = qrb_carve(smem_raw, N, NB, tc);
-  int *perm = reinterpret_cast<int *>(L.Ct);
-  // column order: decreasing norm, ties by index
-  for (int j = wave; j < N; j += nw) {
-    const double *col = A + (size_t)j * N;
-    double s = 0.0;
-    for (int r = lane; r < N; r += 64) s += col[r] * col[r];
-    s = qrb_wave_sum(s);
-    if (lane == 0) L.nrm[j] = s;
+  // column order: decreasing norm, ties by index (norms of 4 columns per wavefront at once)
+  for (int j0 = wave; j0 < N; j0 += 4 * nw) {
+    double sq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + nw * u;
+      const double *col = A + (size_t)(j < N ? j : 0) * N;
+      double v[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        v[t] = (j < N && r < N) ? col[r] : 0.0;
+      }
+      sq[u] = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) sq[u] += v[t] * v[t];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sq[u] = qrb_wave_sum(sq[u]);
+    if (lane == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (j0 + nw * u < N) L.nrm[j0 + nw * u] = sq[u];
   }
   __syncthreads();
   for (int j = threadIdx.x; j < N; j += blockDim.x) {
@@ -325,68 +503,69 @@ __global__ void __launch_bounds__(512) k_qr_blk(QrbArgs a) {
         rank += (y > x) || (y == x && i < j);
       }
     }
-    perm[rank] = j;
+    L.perm[rank] = j;
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < N; j += blockDim.x) ipvt[j] = perm[j] + 1;
-  for (int c = wave; c < N; c += nw) {
-    const double *src = A + (size_t)perm[c] * N;
-    double *dst = W + (size_t)c * N;
-    for (int r = lane; r < N; r += 64) dst[r] = src[r];
-  }
-  __syncthreads();
+  for (int j = threadIdx.x; j < N; j += blockDim.x) ipvt[j] = L.perm[j] + 1;
+  // panels; the first one reads the columns of A through the order, later ones W
   for (int j0 = 0; j0 < N; j0 += NB) {
     const int w = min(NB, N - j0), m = N - j0;
-    for (int idx = threadIdx.x; idx < m * w; idx += blockDim.x) {
-      const int c = idx / m, r = idx - c * m;
-      L.Pn[(size_t)c * L.ld + r] = W[(j0 + r) + (size_t)(j0 + c) * N];
-    }
-    __syncthreads();
-    qrb_panel<NB>(L, m, w, N - 1 - j0);
+    const bool first = j0 == 0;
+    qrb_panel_reg<NB, RPL>(L, first ? A : W + j0 + (size_t)j0 * N, N, first ? L.perm : nullptr, m, w,
+                           N - 1 - j0);
     if (j0 + w < N) {
       qrb_build_t<NB>(L, m, w);
       for (int c0 = j0 + w; c0 < N; c0 += tc) {
         const int tcn = min(tc, N - c0);
-        for (int idx = threadIdx.x; idx < m * tcn; idx += blockDim.x) {
-          const int c = idx / m, r = idx - c * m;
-          L.Ct[(size_t)c * L.ld + r] = W[(j0 + r) + (size_t)(c0 + c) * N];
-        }
+        if (first) qrb_tile_load<RPL>(L.Ct, L.ld, A, N, L.perm + c0, m, tcn);
+        else qrb_tile_load<RPL>(L.Ct, L.ld, W + j0 + (size_t)c0 * N, N, nullptr, m, tcn);
         __syncthreads();
         qrb_apply_tile<NB>(L, m, w, tcn, 1);
-        for (int idx = threadIdx.x; idx < m * tcn; idx += blockDim.x) {
-          const int c = idx / m, r = idx - c * m;
-          W[(j0 + r) + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
-        }
+        // rows of this panel are final R rows: normalised into A; the rest back to W.
+        // (The first panel still reads A's columns through the order: all to W.)
+        for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int r = lane + 64 * t;
+            if (r < m) {
+              const double x = L.Ct[(size_t)c * L.ld + r];
+              if (r < w && !first) A[(j0 + r) + (size_t)(c0 + c) * N] = x / fabs(L.s_beta[r]);
+              else W[(j0 + r) + (size_t)(c0 + c) * N] = x;
+            }
+          }
         __syncthreads();
       }
+      if (first)  // A is consumed: the first panel's R rows of the trailing columns
+        for (int c = w + wave; c < N; c += nw)
+          for (int r = lane; r < w; r += 64) A[r + (size_t)c * N] = W[r + (size_t)c * N] / fabs(L.s_beta[r]);
     }
-    // panel back: beta on the diagonal, reflectors below
-    for (int idx = threadIdx.x; idx < m * w; idx += blockDim.x) {
-      const int c = idx / m, r = idx - c * m;
-      W[(j0 + r) + (size_t)(j0 + c) * N] = (r == c) ? L.s_beta[c] : L.Pn[(size_t)c * L.ld + r];
+    // the panel's columns of A: R rows above (already in A), diagonal block and reflectors
+    for (int c = wave; c < w; c += nw)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        if (r < m) {
+          double x;
+          if (r < c) x = L.Pn[(size_t)c * L.ld + r] / fabs(L.s_beta[r]);
+          else if (r == c) x = L.s_beta[c] / fabs(L.s_beta[c]);
+          else x = L.Pn[(size_t)c * L.ld + r];
+          A[(j0 + r) + (size_t)(j0 + c) * N] = x;
+        }
+      }
+    for (int c = threadIdx.x; c < w; c += blockDim.x) {
+      tau[j0 + c] = L.s_tau[c];
+      D[j0 + c] = fabs(L.s_beta[c]);
     }
-    for (int c = threadIdx.x; c < w; c += blockDim.x) tau[j0 + c] = L.s_tau[c];
     __syncthreads();
-  }
-  // D = |R_ii|; R rows normalised; reflectors copied
-  for (int r = threadIdx.x; r < N; r += blockDim.x) {
-    const double d = fabs(W[r + (size_t)r * N]);
-    D[r] = d;
-    L.nrm[r] = 1.0 / d;
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
-    const int j = idx / N, r = idx - j * N;
-    const double x = W[idx];
-    A[idx] = (r <= j) ? x * L.nrm[r] : x;
   }
 }
 
 // Q = H_0 H_1 ... H_{N-1} (ZUNGQR) by block reflectors from the last panel back.
-template <int NB>
+template <int NB, int RPL>
 __global__ void __launch_bounds__(512) k_orgqr_blk(QrbArgs a) {
   const int N = a.N, tc = a.tc;
   const int chain = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const double *A = a.Ain + (size_t)chain * N * N;
   const double *tau = a.tin + (size_t)chain * N;
   double *Q = a.Q + (size_t)chain * N * N;
@@ -395,10 +574,7 @@ __global__ void __launch_bounds__(512) k_orgqr_blk(QrbArgs a) {
   const int npan = (N + NB - 1) / NB;
   for (int p = npan - 1; p >= 0; --p) {
     const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
-    for (int idx = threadIdx.x; idx < m * w; idx += blockDim.x) {
-      const int c = idx / m, r = idx - c * m;
-      L.Pn[(size_t)c * L.ld + r] = A[(j0 + r) + (size_t)(j0 + c) * N];
-    }
+    qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);
     for (int c = threadIdx.x; c < w; c += blockDim.x) L.s_tau[c] = tau[j0 + c];
     __syncthreads();
     qrb_build_t<NB>(L, m, w);
@@ -406,18 +582,26 @@ __global__ void __launch_bounds__(512) k_orgqr_blk(QrbArgs a) {
       const int tcn = min(tc, N - c0);
       // Q[j0:, c0:c0+tcn]: the panels after this one wrote Q[j0+w:, j0+w:];
       // everything else still is the unit matrix
-      for (int idx = threadIdx.x; idx < m * tcn; idx += blockDim.x) {
-        const int c = idx / m, r = idx - c * m;
-        const int gr = j0 + r, gc = c0 + c;
-        const bool done = gr >= j0 + w && gc >= j0 + w;
-        L.Ct[(size_t)c * L.ld + r] = done ? Q[gr + (size_t)gc * N] : (gr == gc ? 1.0 : 0.0);
+      if (c0 >= j0 + w) {
+        qrb_tile_load<RPL>(L.Ct, L.ld, Q + j0 + (size_t)c0 * N, N, nullptr, m, tcn);
+        for (int c = wave; c < tcn; c += nw)
+          for (int r = lane; r < w && r < m; r += 64) L.Ct[(size_t)c * L.ld + r] = 0.0;
+      } else {
+        for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+          for (int t = 0; t < RPL; ++t) {
+            const int r = lane + 64 * t;
+            if (r < m) L.Ct[(size_t)c * L.ld + r] = (j0 + r == c0 + c) ? 1.0 : 0.0;
+          }
       }
       __syncthreads();
       qrb_apply_tile<NB>(L, m, w, tcn, 0);
-      for (int idx = threadIdx.x; idx < m * tcn; idx += blockDim.x) {
-        const int c = idx / m, r = idx - c * m;
-        Q[(j0 + r) + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
-      }
+      for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+        for (int t = 0; t < RPL; ++t) {
+          const int r = lane + 64 * t;
+          if (r < m) Q[(j0 + r) + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
+        }
       __syncthreads();
     }
   }

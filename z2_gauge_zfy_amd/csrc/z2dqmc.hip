@@ -860,6 +860,7 @@ static int launch_orgqr(int N, dim3 grid, hipStream_t st, const double *A, const
 // panel width and trailing tile of the blocked QR that fit the LDS for order n (0: none)
 static bool qrb_config(size_t lds_max, int n, int *nb, int *tc) {
   static const int cand[3][2] = {{32, 32}, {32, 16}, {16, 16}};
+  if (n > 512) return false;  // register panel: at most 8 rows per lane
   for (auto &k : cand)
     if (qrb_lds_bytes(n, k[0], k[1]) <= lds_max) {
       *nb = k[0];
@@ -870,19 +871,35 @@ static bool qrb_config(size_t lds_max, int n, int *nb, int *tc) {
 }
 
 // one launch of the blocked QR (orgqr = 0) or of its explicit Q (orgqr = 1)
-static int launch_qrb(int orgqr, int nb, int nmat, size_t sh, hipStream_t st, const QrbArgs &q) {
-  const void *fn = orgqr ? (nb == 32 ? (const void *)k_orgqr_blk<32> : (const void *)k_orgqr_blk<16>)
-                         : (nb == 32 ? (const void *)k_qr_blk<32> : (const void *)k_qr_blk<16>);
+template <int NB, int RPL, int ORG>
+static int launch_qrb_t(int nmat, size_t sh, hipStream_t st, const QrbArgs &q) {
+  const void *fn = ORG ? (const void *)k_orgqr_blk<NB, RPL> : (const void *)k_qr_blk<NB, RPL>;
   if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  if (orgqr) {
-    if (nb == 32) hipLaunchKernelGGL(k_orgqr_blk<32>, dim3(nmat), dim3(512), sh, st, q);
-    else hipLaunchKernelGGL(k_orgqr_blk<16>, dim3(nmat), dim3(512), sh, st, q);
-  } else {
-    if (nb == 32) hipLaunchKernelGGL(k_qr_blk<32>, dim3(nmat), dim3(512), sh, st, q);
-    else hipLaunchKernelGGL(k_qr_blk<16>, dim3(nmat), dim3(512), sh, st, q);
-  }
+  if (ORG) hipLaunchKernelGGL((k_orgqr_blk<NB, RPL>), dim3(nmat), dim3(512), sh, st, q);
+  else hipLaunchKernelGGL((k_qr_blk<NB, RPL>), dim3(nmat), dim3(512), sh, st, q);
   LAUNCH_CHECK();
   return Z2_OK;
+}
+
+template <int NB, int ORG>
+static int launch_qrb_nb(int rpl, int nmat, size_t sh, hipStream_t st, const QrbArgs &q) {
+  switch (rpl) {
+    case 1: return launch_qrb_t<NB, 1, ORG>(nmat, sh, st, q);
+    case 2: return launch_qrb_t<NB, 2, ORG>(nmat, sh, st, q);
+    case 3: return launch_qrb_t<NB, 3, ORG>(nmat, sh, st, q);
+    case 4: return launch_qrb_t<NB, 4, ORG>(nmat, sh, st, q);
+    case 5: return launch_qrb_t<NB, 5, ORG>(nmat, sh, st, q);
+    case 6: return launch_qrb_t<NB, 6, ORG>(nmat, sh, st, q);
+    case 7: return launch_qrb_t<NB, 7, ORG>(nmat, sh, st, q);
+    case 8: return launch_qrb_t<NB, 8, ORG>(nmat, sh, st, q);
+  }
+  return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
+}
+
+static int launch_qrb(int orgqr, int nb, int nmat, size_t sh, hipStream_t st, const QrbArgs &q) {
+  const int rpl = (q.N + 63) / 64;
+  if (orgqr) return nb == 32 ? launch_qrb_nb<32, 1>(rpl, nmat, sh, st, q) : launch_qrb_nb<16, 1>(rpl, nmat, sh, st, q);
+  return nb == 32 ? launch_qrb_nb<32, 0>(rpl, nmat, sh, st, q) : launch_qrb_nb<16, 0>(rpl, nmat, sh, st, q);
 }
 
 static double *qr_workspace(z2_ctx *c, int n) { return n == c->N ? c->qrw : (n == 2 * c->N ? c->qrw2 : nullptr); }
