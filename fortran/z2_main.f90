@@ -1,19 +1,24 @@
 ! Driver with the structure of the reference program (src/main.f90:1-137):
-! init, pre, Nbin bins of local sweeps with the per-bin equal-time observable
-! files appended (src/fourier_trans.f90:251-278), final configuration written.
-! All chains of seeds.txt run in this one process on the GPU.
+! init, boson warm-up (is_warm), pre, Nbin bins of [Wolff global sweep
+! (is_global) + local sweep] with the per-bin equal-time observable files
+! appended (src/fourier_trans.f90:251-278), final configuration written.
+! All chains of seeds.txt (or the first N: second argument, the reference's
+! MPI size) run in this one process on the GPU.
+! Usage: z2_gauge_gpu.out [fp64|mixed] [N]
 program z2_gauge_gpu
     use iso_c_binding
     use DQMC_Model_mod
     use ProcessMatrix
     use Stabilize_mod
     use LocalSweep_mod
+    use GlobalUpdate_mod
     implicit none
-    integer :: iseed, nbc
+    integer :: iseed, nbc, nth
     logical :: is_beta, istau_tmp
     integer(kind=8) :: ICPU_1, ICPU_2, N_P_SEC
     real(kind=8) :: xmax, xmean
     type(LocalSweep) :: Sweep_local
+    type(GlobalUpdate) :: Sweep_global
     type(Propagator), allocatable :: PropU, PropD
     type(WrapList), allocatable :: WrU, WrD
     character(len=32) :: arg
@@ -21,6 +26,10 @@ program z2_gauge_gpu
     if (command_argument_count() >= 1) then
         call get_command_argument(1, arg)
         if (trim(arg) == 'mixed') precision_mode = Z2_PREC_MIXED
+    endif
+    if (command_argument_count() >= 2) then
+        call get_command_argument(2, arg)
+        read(arg, *) max_chains
     endif
     call SYSTEM_CLOCK(COUNT_RATE = N_P_SEC)
     call SYSTEM_CLOCK(COUNT = ICPU_1)
@@ -31,14 +40,23 @@ program z2_gauge_gpu
     allocate(WrD); call WrD%make()
     call Stabilize_init()
     call Sweep_local%init()
+    if (is_global) call Sweep_global%init()
     open(unit=50, file='info.txt', status='unknown', action='write')
     write(50, *) 'Lx, Ly, Ltrot, Beta, Nwrap, Nbin, chains  :', Nlx, Nly, Ltrot, Beta, Nwrap, Nbin, n_chains
+    if (is_warm) then
+        do nth = 1, Nwarm
+            call Sweep_local%therm(iseed)
+        enddo
+        call Sweep_local%ctrl_print_t()
+    endif
     call Sweep_local%pre(PropU, PropD, WrU, WrD)
     is_beta = .true.; istau_tmp = .false.
     do nbc = 1, Nbin
+        if (is_global) call Sweep_global%sweep(PropU, PropD, WrU, WrD, iseed, is_beta)
         call Sweep_local%sweep(PropU, PropD, WrU, WrD, iseed, is_beta, istau_tmp)
         call write_obs_equal()
     enddo
+    if (is_global) call Sweep_global%ctrl_print()
     call Sweep_local%ctrl_print_l(istau_tmp)
     xmax = PropU%Xmaxm
     xmean = PropU%Xmeanm / dble(2 * Nbin * Nst * Nsweep)
@@ -48,6 +66,7 @@ program z2_gauge_gpu
     write(50, *) 'Tot CPU time                                   :', dble(ICPU_2 - ICPU_1) / dble(N_P_SEC)
     close(50)
     call Sweep_local%clear()
+    if (is_global) call Sweep_global%clear()
     call Stabilize_clear()
     call Model_clear(iseed)
 contains

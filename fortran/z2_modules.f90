@@ -12,8 +12,10 @@ module DQMC_Model_mod
     integer :: Nwrap, Nbin, Nsweep, Nthermal, Nwarm, Nglobal, absolute, Lq, Ndim, Nst
     logical :: is_tau, is_warm, is_global
     integer :: precision_mode = 0
-    ! all chains of this process (one reference MPI rank each)
+    ! all chains of this process (one reference MPI rank each); max_chains > 0
+    ! takes only the first max_chains seeds (the reference's MPI size)
     integer :: n_chains = 1
+    integer :: max_chains = 0
     type(c_ptr) :: ctx = c_null_ptr
 contains
     ! read_input + Params_set (src/calc_basic.f90:50-102)
@@ -45,9 +47,8 @@ contains
         integer, allocatable :: seeds(:)
         integer :: k, ios, s, conf0
         call read_input()
-        if (is_global .or. is_warm .or. is_tau) then
-            write(6, *) "note: Wolff global updates, boson warm-up and time-displaced measurements", &
-                        " are not part of this engine build; running local sweeps only"
+        if (is_tau) then
+            write(6, *) "note: this driver writes no time-displaced output files; is_tau is ignored"
         endif
         if (NlxTherm /= Nlx .or. NlyTherm /= Nly .or. LtrotTherm /= Ltrot) then
             write(6, *) "thermalisation-size transfer is not supported by this driver"; stop 1
@@ -66,7 +67,7 @@ contains
             if (ios /= 0) exit
             n_chains = n_chains + 1
             seeds(n_chains) = s
-            if (n_chains == size(seeds)) exit
+            if (n_chains == size(seeds) .or. n_chains == max_chains) exit
         enddo
         close(10)
         if (n_chains == 0) then
@@ -83,20 +84,25 @@ contains
         call z2_check(z2_get_seed(ctx, 0_c_int32_t, iseed), "z2_get_seed")
     end subroutine Model_init
 
-    ! Model_clear -> conf_out (src/model.f90:38-45, src/fields.f90:509-557)
+    ! Model_clear -> conf_out (src/model.f90:38-45, src/fields.f90:509-557):
+    ! seed, sigma(ii, nt), lambda(i) of every chain
     subroutine Model_clear(iseed)
         integer, intent(in) :: iseed
-        real(c_double), allocatable :: sig(:)
+        real(c_double), allocatable :: sig(:), lam(:)
         integer(c_int32_t) :: s
         integer :: k, i
-        allocate(sig(2 * Lq * Ltrot))
+        allocate(sig(2 * Lq * Ltrot), lam(Lq))
         open(unit=35, file='confout.txt', status='unknown')
         do k = 0, n_chains - 1
             call z2_check(z2_get_seed(ctx, int(k, c_int32_t), s), "z2_get_seed")
             call z2_check(z2_get_sigma(ctx, int(k, c_int32_t), sig), "z2_get_sigma")
+            call z2_check(z2_get_lambda(ctx, int(k, c_int32_t), lam), "z2_get_lambda")
             write(35, *) s
             do i = 1, size(sig)
                 write(35, *) sig(i)
+            enddo
+            do i = 1, size(lam)
+                write(35, *) lam(i)
             enddo
         enddo
         close(35)
@@ -237,7 +243,9 @@ module LocalSweep_mod
         procedure :: clear => Local_sweep_clear
         procedure :: pre => Local_sweep_pre
         procedure :: sweep => Local_sweep
+        procedure :: therm => Local_sweep_therm
         procedure, nopass :: ctrl_print_l => Local_control_print
+        procedure, nopass :: ctrl_print_t => Therm_control_print
     end type LocalSweep
     type(ObserEqual), allocatable :: Obs_equal
 contains
@@ -257,7 +265,17 @@ contains
         class(WrapList), intent(inout) :: WrU, WrD
         call z2_check(z2_pre(ctx), "z2_pre")
     end subroutine Local_sweep_pre
-    ! src/local_sweep.f90:152-195 (all chains; iseed returns chain 0's state)
+    ! Local_sweep_therm (src/local_sweep.f90:73-85), all chains
+    subroutine Local_sweep_therm(this, iseed)
+        class(LocalSweep), intent(inout) :: this
+        integer, intent(inout) :: iseed
+        integer(c_int32_t) :: s
+        call z2_check(z2_therm(ctx), "z2_therm")
+        call z2_check(z2_get_seed(ctx, 0_c_int32_t, s), "z2_get_seed")
+        iseed = s
+    end subroutine Local_sweep_therm
+    ! src/local_sweep.f90:152-195 (all chains, each in its own direction: the
+    ! engine keeps every chain's is_beta; is_beta and iseed return chain 0's)
     subroutine Local_sweep(this, PropU, PropD, WrU, WrD, iseed, is_beta, toggle)
         class(LocalSweep), intent(inout) :: this
         class(Propagator), intent(inout) :: PropU, PropD
@@ -267,9 +285,9 @@ contains
         logical, intent(in) :: toggle
         integer(c_int32_t) :: b, s
         real(c_double) :: xmax, xmean
-        b = 0
-        if (is_beta) b = 1
-        call z2_check(z2_sweep(ctx, b), "z2_sweep")
+        call z2_check(z2_sweep(ctx, Z2_BETA_PER_CHAIN), "z2_sweep")
+        call z2_check(z2_get_is_beta(ctx, 0_c_int32_t, b), "z2_get_is_beta")
+        is_beta = b /= 0
         call Obs_equal%ave(2 * Ltrot * Nsweep)
         call z2_check(z2_get_seed(ctx, 0_c_int32_t, s), "z2_get_seed")
         iseed = s
@@ -289,6 +307,70 @@ contains
             acc = acc + cnt(1); prop = prop + cnt(2)
         enddo
         write(50, *) 'Accept_Klocal_shift                            :', dble(acc) / dble(max(prop, 1_c_int64_t))
+        write(50, *) 'Accept_lambda_local                            :', 0.d0
         if (toggle) continue
     end subroutine Local_control_print
+    ! src/local_sweep.f90:215-227
+    subroutine Therm_control_print()
+        integer(c_int64_t) :: cnt(2), acc, prop
+        integer :: k
+        if (Nwarm <= 0) return
+        acc = 0; prop = 0
+        do k = 0, n_chains - 1
+            call z2_check(z2_get_therm_counters(ctx, int(k, c_int32_t), cnt), "z2_get_therm_counters")
+            acc = acc + cnt(1); prop = prop + cnt(2)
+        enddo
+        write(50, *) 'Thermalize Accept Ratio                        :', dble(acc) / dble(max(prop, 1_c_int64_t))
+    end subroutine Therm_control_print
 end module LocalSweep_mod
+
+module GlobalUpdate_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod
+    use ProcessMatrix
+    implicit none
+    ! src/global_update.f90:183-354 (Wolff cluster update of a bond's time line)
+    type :: GlobalUpdate
+    contains
+        procedure :: init => Global_init
+        procedure :: clear => Global_clear
+        procedure :: sweep => Global_sweep
+        procedure, nopass :: ctrl_print => Global_control_print
+    end type GlobalUpdate
+contains
+    subroutine Global_init(this)
+        class(GlobalUpdate), intent(inout) :: this
+    end subroutine Global_init
+    subroutine Global_clear(this)
+        class(GlobalUpdate), intent(inout) :: this
+    end subroutine Global_clear
+    ! Global_sweep (src/global_update.f90:313-337): Nglobal trials per chain in the
+    ! chain's own direction; is_beta and iseed return chain 0's
+    subroutine Global_sweep(this, PropU, PropD, WrU, WrD, iseed, is_beta)
+        class(GlobalUpdate), intent(inout) :: this
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(WrapList), intent(inout) :: WrU, WrD
+        integer, intent(inout) :: iseed
+        logical, intent(inout) :: is_beta
+        integer(c_int32_t) :: b, s
+        call z2_check(z2_global_sweep(ctx, int(Nglobal, c_int32_t)), "z2_global_sweep")
+        call z2_check(z2_get_is_beta(ctx, 0_c_int32_t, b), "z2_get_is_beta")
+        is_beta = b /= 0
+        call z2_check(z2_get_seed(ctx, 0_c_int32_t, s), "z2_get_seed")
+        iseed = s
+    end subroutine Global_sweep
+    ! src/global_update.f90:339-354, averaged over the chains
+    subroutine Global_control_print()
+        integer(c_int64_t) :: cnt(3), acc, prop
+        real(c_double) :: sz, szsum
+        integer :: k
+        acc = 0; prop = 0; szsum = 0.d0
+        do k = 0, n_chains - 1
+            call z2_check(z2_get_global_counters(ctx, int(k, c_int32_t), cnt, sz), "z2_get_global_counters")
+            acc = acc + cnt(1); prop = prop + cnt(2); szsum = szsum + sz
+        enddo
+        write(50, *) 'Average size of Wolff cluster                  :', szsum / dble(n_chains * Nbin)
+        write(50, *) 'Accept_Kglobal                                 :', dble(acc) / dble(max(prop, 1_c_int64_t))
+    end subroutine Global_control_print
+end module GlobalUpdate_mod

@@ -6,6 +6,7 @@ module z2dqmc_c
     implicit none
 
     integer(c_int32_t), parameter :: Z2_PREC_FP64 = 0, Z2_PREC_MIXED = 1
+    integer(c_int32_t), parameter :: Z2_BETA_PER_CHAIN = -1
 
     type, bind(C) :: z2_params
         integer(c_int32_t) :: Nlx, Nly, Ltrot, Nwrap, Nsweep, precision
@@ -133,6 +134,47 @@ module z2dqmc_c
             type(c_ptr), value :: ctx
             integer(c_int32_t), value :: chain
             real(c_double), intent(out) :: G(*)
+            integer(c_int) :: rc
+        end function
+        function z2_get_lambda(ctx, chain, lam) bind(C, name="z2_get_lambda") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: lam(*)
+            integer(c_int) :: rc
+        end function
+        function z2_global_sweep(ctx, nglobal) bind(C, name="z2_global_sweep") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nglobal
+            integer(c_int) :: rc
+        end function
+        function z2_get_is_beta(ctx, chain, v) bind(C, name="z2_get_is_beta") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            integer(c_int32_t), intent(out) :: v
+            integer(c_int) :: rc
+        end function
+        function z2_get_global_counters(ctx, chain, out, size_cluster) bind(C, name="z2_get_global_counters") &
+                result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_int64_t, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            integer(c_int64_t), intent(out) :: out(3)
+            real(c_double), intent(out) :: size_cluster
+            integer(c_int) :: rc
+        end function
+        function z2_therm(ctx) bind(C, name="z2_therm") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_get_therm_counters(ctx, chain, out) bind(C, name="z2_get_therm_counters") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_int64_t
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            integer(c_int64_t), intent(out) :: out(2)
             integer(c_int) :: rc
         end function
         function c_strlen(s) bind(C, name="strlen") result(n)

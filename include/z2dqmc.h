@@ -66,12 +66,41 @@ int z2_init_fields(z2_ctx *ctx);
 int z2_set_sigma(z2_ctx *ctx, int32_t chain, const double *sigma, int32_t seed);
 int z2_get_sigma(z2_ctx *ctx, int32_t chain, double *sigma);
 int z2_get_seed(z2_ctx *ctx, int32_t chain, int32_t *seed);
+/* lambda(N) of one chain (SigmaConf%lambda, src/fields.f90:10; drawn by
+ * conf_set, lambda_random_fill + lambda_enforce_sector :559-603), kept for the
+ * confin / confout files; the sweeps do not use it */
+int z2_get_lambda(z2_ctx *ctx, int32_t chain, double *lambda);
+int z2_set_lambda(z2_ctx *ctx, int32_t chain, const double *lambda);
 
 /* LocalSweep%pre (src/local_sweep.f90:87-104) */
 int z2_pre(z2_ctx *ctx);
 /* LocalSweep%sweep (src/local_sweep.f90:152-195): Nsweep x (left + right pass)
- * with equal-time observables at every slice; is_beta selects the pass order. */
+ * with equal-time observables at every slice; is_beta selects the pass order
+ * (1: left pass first) for all chains, or Z2_BETA_PER_CHAIN takes each chain's
+ * own is_beta (reference: one is_beta per MPI rank, src/main.f90:73-77). */
+#define Z2_BETA_PER_CHAIN (-1)
 int z2_sweep(z2_ctx *ctx, int32_t is_beta);
+/* a chain's is_beta: 1 after z2_pre, set by z2_sweep(ctx, 0 / 1), reversed by
+ * an accepted Wolff trial */
+int z2_set_is_beta(z2_ctx *ctx, int32_t chain, int32_t is_beta);
+int z2_get_is_beta(z2_ctx *ctx, int32_t chain, int32_t *is_beta);
+
+/* Global_sweep (src/global_update.f90:313-337, Wolff_flip :65-181,
+ * ratioK_fermion and GlobalK_prop_L/R src/globalK.f90:15-151): Nglobal trials
+ * per chain, each flipping one bond along a time cluster; the trial propagates
+ * in the chain's own direction (Global_sweep_L when is_beta = 1, else _R) and an
+ * accepted trial reverses its is_beta.  Run before z2_sweep(ctx,
+ * Z2_BETA_PER_CHAIN) as the reference's main does (src/main.f90:76-77). */
+int z2_global_sweep(z2_ctx *ctx, int32_t Nglobal);
+/* cumulative Wolff counts: accepted, proposed, sum of cluster sizes; and the
+ * running wolff_size_cluster (sum over global sweeps of the mean cluster size) */
+int z2_get_global_counters(z2_ctx *ctx, int32_t chain, uint64_t out[3], double *size_cluster);
+/* one LocalK_therm sweep (src/localK.f90:400-420): boson-only Metropolis over
+ * every (nt, ii), nt outer (Local_sweep_therm, src/local_sweep.f90:73-85), for
+ * the warm-up loop of src/main.f90:57-58 (is_warm) */
+int z2_therm(z2_ctx *ctx);
+/* cumulative warm-up counts: accepted, proposed */
+int z2_get_therm_counters(z2_ctx *ctx, int32_t chain, uint64_t out[2]);
 
 /* time-displaced measurement (is_tau = 1 in the reference's parameter file):
  * z2_enable_tau allocates its state (about 26 N^2 doubles per chain);

@@ -116,6 +116,7 @@ def lib():
             "z2o_get_dyn_drift": (None, [V, PD, PD]),
             "z2o_get_min_gap": (D, [V]),
             "z2o_reset_min_gap": (None, [V]),
+            "z2o_get_gmin_gap": (D, [V]),
             "z2o_set_decision_log": (None, [V, C.POINTER(C.c_uint32), I64]),
             "z2o_get_decision_count": (I64, [V]),
             "z2o_run_program": (None, [V, C.POINTER(ObsEqual)]),
@@ -385,6 +386,11 @@ class Chain:
 
     def reset_min_gap(self):
         lib().z2o_reset_min_gap(self.ptr)
+
+    @property
+    def global_min_gap(self):
+        """smallest |min(log ratio, 0) - log(r)| over the global accept/reject decisions"""
+        return lib().z2o_get_gmin_gap(self.ptr)
 
     def set_decision_log(self, cap: int):
         self._log = np.zeros(cap, np.uint32)

@@ -933,6 +933,7 @@ struct z2o_chain {
   double *Gtmp, *work;
   int *tau_mask, *queue;
   double min_gap;
+  double gmin_gap; /* smallest |min(total,0) - log(r)| of the global decisions */
   uint32_t *log_buf;
   int64_t log_cap, log_n;
 };
@@ -1134,6 +1135,7 @@ static z2o_chain *chain_alloc(const z2o_model *m) {
   c->tau_mask = xcalloc((size_t)c->T, sizeof(int));
   c->queue = xcalloc((size_t)c->T, sizeof(int));
   c->min_gap = INFINITY;
+  c->gmin_gap = INFINITY;
   if (m->p.is_global) {
     prop_make(&c->gprop, c->N);
     wr_make(&c->gwr, c->N, c->Nst);
@@ -2000,6 +2002,8 @@ static void global_accept_reject(z2o_chain *c, double total, int beta_if_accept)
   double r = z2o_ranf(&c->seed);
   double lhs = total < 0.0 ? total : 0.0;
   double rr = r > RATIO_EPS ? r : RATIO_EPS;
+  double gap = fabs(lhs - log(rr));
+  if (gap < c->gmin_gap) c->gmin_gap = gap;
   c->n_glob_prop += 1;
   size_t ns = sizeof(double) * (size_t)c->N2 * c->T;
   if (lhs >= log(rr)) {
@@ -2141,6 +2145,7 @@ void z2o_get_dyn_drift(const z2o_chain *c, double *xmax3, double *xmean3) {
 }
 double z2o_get_min_gap(const z2o_chain *c) { return c->min_gap; }
 void z2o_reset_min_gap(z2o_chain *c) { c->min_gap = INFINITY; }
+double z2o_get_gmin_gap(const z2o_chain *c) { return c->gmin_gap; }
 void z2o_set_decision_log(z2o_chain *c, uint32_t *buf, int64_t cap) {
   c->log_buf = buf;
   c->log_cap = cap;

@@ -135,6 +135,7 @@ void z2o_get_dyn_drift(const z2o_chain *c, double *xmax3, double *xmean3);
 /* smallest | |ratio| - r | met by any local proposal since the last reset */
 double z2o_get_min_gap(const z2o_chain *c);
 void z2o_reset_min_gap(z2o_chain *c);
+double z2o_get_gmin_gap(const z2o_chain *c);
 /* optional per-proposal log: (bond, slice, accept) packed as
  * bond | slice<<16 | accept<<31 into a caller buffer; cap entries */
 void z2o_set_decision_log(z2o_chain *c, uint32_t *buf, int64_t cap);

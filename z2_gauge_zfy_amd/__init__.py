@@ -4,5 +4,6 @@ The compute path is libz2dqmc.so (hand-written HIP kernels behind the C ABI in
 include/z2dqmc.h); this package is the Python host mirror of the reference's
 sweep driver API.
 """
-from ._lib import Z2_PREC_FP64, Z2_PREC_MIXED, Z2_QR_BLOCKED, Z2_QR_PIVOTED, Z2Error, lib  # noqa: F401
+from ._lib import (Z2_BETA_PER_CHAIN, Z2_PREC_FP64, Z2_PREC_MIXED, Z2_QR_BLOCKED, Z2_QR_PIVOTED,  # noqa: F401
+                   Z2Error, lib)
 from .engine import ChainSpec, Engine, OBS_NAMES  # noqa: F401

@@ -16,6 +16,7 @@ Z2_PREC_FP64 = 0
 Z2_PREC_MIXED = 1
 Z2_QR_BLOCKED = 0
 Z2_QR_PIVOTED = 1
+Z2_BETA_PER_CHAIN = -1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
@@ -74,6 +75,15 @@ _SIGS = {
                                C.POINTER(C.c_double)]),
     "z2_get_obs_equal": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    "z2_get_lambda": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_set_lambda": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_set_is_beta": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_get_is_beta": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
+    "z2_global_sweep": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_get_global_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_double)]),
+    "z2_therm": (C.c_int, [C.c_void_p]),
+    "z2_get_therm_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "z2_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "z2_set_delay": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_set_qr_mode": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -36,6 +36,13 @@ struct FlipArgs {
   uint32_t *record;       // optional: decisions taken, same layout
   int dwords;
   LatticeDev lat;
+  // Wolff trial pass (ratioK_fermion, src/globalK.f90:15-109): when gbond is set
+  // the flip of bond gbond[chain] is applied at the slices where gmask is 1 and
+  // every other proposal is rejected; no random number is drawn, no counter
+  // moves, and log max(|det|^2, 1e-300) of each applied flip is added to glr[2*chain]
+  const int *gbond;       // per chain, 0-based
+  const int *gmask;       // per chain T
+  double *glr;            // per chain [2]
 };
 
 template <typename R>
@@ -148,6 +155,8 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
     s_kmin[1] = 2 * N2 + 2;
   }
   const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
+  // forced bond of a Wolff trial at this slice (-1: none)
+  const int gb = a.gbond ? (a.gmask[(size_t)chain * a.T + (a.nt - 1)] ? a.gbond[chain] : -1) : -1;
   __syncthreads();
   // per-thread proposals k = tid + m * nth: bond, sites, partner bonds, field,
   // random number, 2x2 block (G_p1p1, G_p2p2, G_p1p2, G_p2p1), all in registers
@@ -274,7 +283,8 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       int dT = 2 * s_new * rss[m];
       const double rb = bos[(dS / 4 + 1) * 3 + (dT / 4 + 1)];
       const double ratio = (det * det) * rb;
-      const bool acc = frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rrn[m];
+      const bool acc = a.gbond ? rb_[m] == gb
+                               : (frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rrn[m]);
       if (acc) cand = 2 * k + (s_old > 0 ? 1 : 0);
     }
     cand = wave_min_int(cand);
@@ -314,6 +324,11 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       Pi21 = (-P21) / det;
       Pi12 = (-P12) / det;
       Pi22 = P11 / det;
+      if (a.gbond && tid == 0) {
+        double r = fabs(det * det);
+        if (r < 1.0e-300) r = 1.0e-300;
+        a.glr[2 * chain] = a.glr[2 * chain] + log(r > 1.0e-300 ? r : 1.0e-300);
+      }
     }
     if (tid == 0) {
       sg[b] = (int8_t)(-s_old);
@@ -399,9 +414,11 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   if (tid == 0) {
     a.nacc[chain] = na;
     a.mu_done[chain] = mu_done;
-    uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[N2];
-    a.seed[chain] = (int32_t)(uint32_t)(x & 0x7fffffffull);
-    a.acc_count[2 * chain] += (unsigned long long)tot;
-    a.acc_count[2 * chain + 1] += (unsigned long long)N2;
+    if (!a.gbond) {
+      uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[N2];
+      a.seed[chain] = (int32_t)(uint32_t)(x & 0x7fffffffull);
+      a.acc_count[2 * chain] += (unsigned long long)tot;
+      a.acc_count[2 * chain + 1] += (unsigned long long)N2;
+    }
   }
 }

@@ -15,6 +15,8 @@ struct ChainConst {
   double ep, em;          // exp(+dtau*mu), exp(-dtau*mu)
   double beta;
   double rt;              // hopping t (current-current correlator prefactor)
+  double ptime;           // Wolff: probability to bind equal neighbouring time links
+  double walpha;          // Wolff: dtau * J of the plaquette log ratio
 };
 
 // lattice tables shared by all chains of one context (same L)

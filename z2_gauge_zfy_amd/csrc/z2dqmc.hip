@@ -8,11 +8,14 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
 #include "../../include/z2dqmc.h"
 #include "flip_kernels.hpp"
+#include "global_kernels.hpp"
 #include "kin_kernels.hpp"
 #include "linalg_kernels.hpp"
 #include "obs_kernels.hpp"
@@ -56,9 +59,25 @@ static double ranf_host(int32_t *iq) {
   return (double)s / 2147483648.0;
 }
 
+// One per-chain device array of a context: `bytes` per chain, `slots` slot-major
+// copies (slot stride = all chains).  shift() moves the array's pointer by a
+// number of chains, so that the launch code sees a contiguous range of chains as
+// a whole context.
+enum : int { CF_PERSIST = 1, CF_TRIAL = 2 };  // moves with its chain; changed by a Wolff trial
+struct ChainField {
+  std::function<char *()> get;
+  std::function<void(long)> shift;
+  size_t bytes = 0;
+  int slots = 1;
+  int flags = 0;
+  char *backup = nullptr;  // Wolff trial backup (CF_TRIAL), all chains
+};
+
 struct z2_ctx {
   z2_params p;
-  int N = 0, N2 = 0, T = 0, w = 0, Nst = 0, nch = 0, Lx = 0, Ly = 0;
+  // nch: chains the launch code acts on (all chains, or the range of a ChainRange);
+  // nch_all: chains of the context
+  int N = 0, N2 = 0, T = 0, w = 0, Nst = 0, nch = 0, nch_all = 0, Lx = 0, Ly = 0;
   hipStream_t stream = nullptr;
   std::vector<z2_chain_desc> desc;
   std::vector<ChainConst> hcc;
@@ -107,6 +126,20 @@ struct z2_ctx {
   int *ipvt2 = nullptr;
   double *otau = nullptr;       // [chain][7][T][N]
   double *dyn_drift = nullptr;  // [chain][6]: max of G0t, Gt0, Gtt drifts, then their sums
+  // Wolff global update: trial bond, cluster mask and queue, log ratios, accept
+  // flag, counters; boson warm-up counters
+  int *gbond = nullptr, *gmask = nullptr, *gqueue = nullptr, *gacc = nullptr;
+  double *glr = nullptr, *gsz = nullptr;
+  unsigned long long *gcnt = nullptr, *tcnt = nullptr;
+  bool gmode = false;      // the flip kernel runs in forced (Wolff trial) mode
+  bool gbackup = false;    // trial backups allocated
+  // chains are stored grouped by pass direction (is_beta 1 first); phys maps a
+  // chain's index to its storage slot, isb is the direction per slot
+  std::vector<int> phys, isb;
+  std::vector<double> lam;  // lambda(N) per storage slot (host only: the sweep does not use it)
+  std::vector<ChainField> fields;
+  int *d_perm = nullptr;
+  char *reorder_tmp = nullptr;
   std::vector<void *> allocs;
   bool have_fields = false, have_pre = false;
   // kernel timing hooks: HIP events around every launch of one kernel class
@@ -154,6 +187,98 @@ static void free_ctx(z2_ctx *c) {
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
+
+template <typename T>
+static void reg_field(z2_ctx *c, T *&ptr, size_t per_chain, int slots, int flags) {
+  ChainField f;
+  f.get = [&ptr]() { return reinterpret_cast<char *>(ptr); };
+  f.shift = [&ptr, per_chain](long dc) {
+    if (ptr) ptr += dc * (long)per_chain;
+  };
+  f.bytes = per_chain * sizeof(T);
+  f.slots = slots;
+  f.flags = flags;
+  c->fields.push_back(f);
+}
+
+// every per-chain device array with its per-chain size (arrays allocated later,
+// e.g. by z2_enable_tau, are registered while still null)
+static void register_fields(z2_ctx *c) {
+  const size_t N = c->N, nn = N * N, T = c->T, N2 = c->N2, us = (size_t)c->upd_stride;
+  const int L = c->Nst + 1, P = CF_PERSIST, PT = CF_PERSIST | CF_TRIAL;
+  reg_field(c, c->G, nn, 1, PT);
+  reg_field(c, c->UUR, nn, 1, PT);
+  reg_field(c, c->VUR, nn, 1, PT);
+  reg_field(c, c->UUL, nn, 1, PT);
+  reg_field(c, c->VUL, nn, 1, PT);
+  reg_field(c, c->DUR, N, 1, PT);
+  reg_field(c, c->DUL, N, 1, PT);
+  reg_field(c, c->URl, nn, L, PT);
+  reg_field(c, c->VRl, nn, L, PT);
+  reg_field(c, c->ULl, nn, L, PT);
+  reg_field(c, c->VLl, nn, L, PT);
+  reg_field(c, c->DRl, N, L, PT);
+  reg_field(c, c->DLl, N, L, PT);
+  reg_field(c, c->sig, T * N2, 1, PT);
+  reg_field(c, c->seed, 1, 1, P);
+  reg_field(c, c->Tbuf, us, 1, 0);
+  reg_field(c, c->Vbuf, us, 1, 0);
+  reg_field(c, c->nacc, 1, 1, 0);
+  reg_field(c, c->mu_done, 1, 1, 0);
+  reg_field(c, c->accc, 2, 1, P);
+  reg_field(c, c->xmax, 1, 1, PT);
+  reg_field(c, c->xmean, 1, 1, PT);
+  reg_field(c, c->obs_acc, OBS_NACC, 1, P);
+  reg_field(c, c->obs_res, RES_N, 1, P);
+  for (double **m : {&c->M, &c->Q, &c->X, &c->Y, &c->Gn, &c->qrw}) reg_field(c, *m, nn, 1, 0);
+  reg_field(c, c->tau, N, 1, 0);
+  reg_field(c, c->Dtmp, N, 1, 0);
+  reg_field(c, c->ipvt, N, 1, 0);
+  reg_field(c, c->rec, (size_t)c->dwords, 1, 0);
+  reg_field(c, c->cmp, 1, 1, 0);
+  reg_field(c, c->Gf, nn, 1, PT);
+  reg_field(c, c->UURf, nn, 1, PT);
+  reg_field(c, c->UULf, nn, 1, PT);
+  reg_field(c, c->Tbf, us, 1, 0);
+  reg_field(c, c->Vbf, us, 1, 0);
+  reg_field(c, c->dcc, 1, 1, P);
+  for (double **m : {&c->dUUR, &c->dVUR, &c->dUUL, &c->dVUL, &c->G00, &c->G0t, &c->Gt0, &c->Gtt, &c->iVR, &c->iVL})
+    reg_field(c, *m, nn, 1, 0);
+  for (double **m : {&c->bigA, &c->bigQ, &c->bigZ, &c->bigR, &c->qrw2}) reg_field(c, *m, 4 * nn, 1, 0);
+  reg_field(c, c->dDUR, N, 1, 0);
+  reg_field(c, c->dDUL, N, 1, 0);
+  reg_field(c, c->tau2, 2 * N, 1, 0);
+  reg_field(c, c->D2, 2 * N, 1, 0);
+  reg_field(c, c->ipvt2, 2 * N, 1, 0);
+  reg_field(c, c->otau, (size_t)TAU_NOBS * T * N, 1, P);
+  reg_field(c, c->dyn_drift, 6, 1, P);
+  reg_field(c, c->gbond, 1, 1, 0);
+  reg_field(c, c->gmask, T, 1, 0);
+  reg_field(c, c->gqueue, T, 1, 0);
+  reg_field(c, c->glr, 2, 1, 0);
+  reg_field(c, c->gacc, 1, 1, 0);
+  reg_field(c, c->gcnt, 3, 1, P);
+  reg_field(c, c->gsz, 2, 1, P);
+  reg_field(c, c->tcnt, 2, 1, P);
+}
+
+// A contiguous range [c0, c0 + cnt) of storage slots seen by the launch code as
+// the whole context (every per-chain pointer moved by c0 chains), on `stream`.
+struct ChainRange {
+  z2_ctx *c;
+  int c0, nsave;
+  hipStream_t ssave;
+  ChainRange(z2_ctx *ctx, int first, int cnt, hipStream_t st) : c(ctx), c0(first), nsave(ctx->nch), ssave(ctx->stream) {
+    for (auto &f : c->fields) f.shift(c0);
+    c->nch = cnt;
+    c->stream = st;
+  }
+  ~ChainRange() {
+    for (auto &f : c->fields) f.shift(-c0);
+    c->nch = nsave;
+    c->stream = ssave;
+  }
+};
 
 // per-chain constants with the reference's libm calls
 static ChainConst make_const(const z2_chain_desc &d, int Ltrot) {
@@ -204,6 +329,19 @@ static ChainConst make_const(const z2_chain_desc &d, int Ltrot) {
   k.em = exp(-Dtau * d.mu);
   k.beta = d.Beta;
   k.rt = d.RT;
+  // compute_time_bond_prob (src/global_update.f90:88-100) and the plaquette
+  // coupling of plaquette_log_ratio (:151-172)
+  double pt = 0.0;
+  if (fabs(d.h) > 1.0e-10) {
+    double ta = tanh(Dtau * fabs(d.h));
+    if (ta < 1.0e-300) ta = 1.0e-300;
+    double g = -0.5 * log(ta);
+    pt = 1.0 - exp(-2.0 * g);
+  }
+  if (pt > 1.0) pt = 1.0;
+  if (pt < 0.0) pt = 0.0;
+  k.ptime = pt;
+  k.walpha = Dtau * d.J;
   return k;
 }
 
@@ -229,6 +367,10 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
   c->w = p->Nwrap;
   c->Nst = p->Ltrot / p->Nwrap;
   c->nch = n_chains;
+  c->nch_all = n_chains;
+  for (int i = 0; i < n_chains; ++i) c->phys.push_back(i);
+  c->isb.assign(n_chains, 1);
+  c->lam.assign((size_t)n_chains * c->N, 1.0);
   c->desc.assign(chains, chains + n_chains);
   if (c->p.Nsweep < 1) c->p.Nsweep = 1;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -403,6 +545,18 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
         if (hipMemcpy(m, If.data(), sizeof(float) * B * nn, hipMemcpyHostToDevice) != hipSuccess) rc = Z2_EHIP;
       if (rc) { rc = set_err(Z2_EHIP, "init copies (fp32)"); break; }
     }
+    // Wolff and warm-up state
+    if ((rc = dalloc(c, &c->gbond, B)) || (rc = dalloc(c, &c->gmask, B * c->T)) ||
+        (rc = dalloc(c, &c->gqueue, B * c->T)) || (rc = dalloc(c, &c->glr, 2 * B)) || (rc = dalloc(c, &c->gacc, B)) ||
+        (rc = dalloc(c, &c->gcnt, 3 * B)) || (rc = dalloc(c, &c->gsz, 2 * B)) || (rc = dalloc(c, &c->tcnt, 2 * B)) ||
+        (rc = dalloc(c, &c->d_perm, B)))
+      break;
+    if (hipMemset(c->gcnt, 0, sizeof(unsigned long long) * 3 * B) || hipMemset(c->gsz, 0, sizeof(double) * 2 * B) ||
+        hipMemset(c->tcnt, 0, sizeof(unsigned long long) * 2 * B) || hipMemset(c->gacc, 0, sizeof(int) * B)) {
+      rc = set_err(Z2_EHIP, "memset");
+      break;
+    }
+    register_fields(c);
     std::vector<int32_t> seeds(B);
     for (size_t b = 0; b < B; ++b) seeds[b] = chains[b].seed;
     if (hipMemcpy(c->seed, seeds.data(), sizeof(int32_t) * B, hipMemcpyHostToDevice) != hipSuccess) { rc = set_err(Z2_EHIP, "seed"); break; }
@@ -457,7 +611,8 @@ extern "C" int z2_sync(z2_ctx *c) {
 
 // ---------------------------------------------------------------- fields
 // conf_set, src/fields.f90:311-427 (therm size == run size: conf_transfer is the identity)
-static void conf_set_host(int Lx, int Ly, int T, int absolute, int32_t *seed, std::vector<double> &sig) {
+static void conf_set_host(int Lx, int Ly, int T, int absolute, double Q, int32_t *seed, std::vector<double> &sig,
+                          double *lam) {
   const int N = Lx * Ly, N2 = 2 * N;
   sig.assign((size_t)N2 * T, 1.0);
   if (absolute == 1) {
@@ -483,7 +638,13 @@ static void conf_set_host(int Lx, int Ly, int T, int absolute, int32_t *seed, st
       for (int nt = 0; nt < T; ++nt) sig[(size_t)nt * N2 + ii] = v;
     }
   }
-  for (int i = 0; i < N; ++i) (void)ranf_host(seed);  // lambda_random_fill draws, src/fields.f90:559-578
+  // lambda_random_fill + lambda_enforce_sector (src/fields.f90:559-603)
+  double prod = 1.0;
+  for (int i = 0; i < N; ++i) {
+    lam[i] = ranf_host(seed) >= 0.5 ? 1.0 : -1.0;
+    prod = prod * lam[i];
+  }
+  if (prod * (Q >= 0.0 ? 1.0 : -1.0) < 0.0) lam[N - 1] = -lam[N - 1];
 }
 
 static int upload_sigma(z2_ctx *c, int chain, const double *sigma) {
@@ -505,7 +666,7 @@ extern "C" int z2_init_fields(z2_ctx *c) {
     int32_t s = c->desc[b].seed;
     int ab = c->desc[b].absolute;
     if (ab < 1 || ab > 5) return set_err(Z2_EINVAL, "absolute must be 1..5");
-    conf_set_host(c->Lx, c->Ly, c->T, ab, &s, sig);
+    conf_set_host(c->Lx, c->Ly, c->T, ab, c->desc[b].Q, &s, sig, c->lam.data() + (size_t)b * c->N);
     CHK(upload_sigma(c, b, sig.data()));
     seeds[b] = s;
   }
@@ -515,7 +676,8 @@ extern "C" int z2_init_fields(z2_ctx *c) {
 }
 
 extern "C" int z2_set_sigma(z2_ctx *c, int32_t chain, const double *sigma, int32_t seed) {
-  if (!c || !sigma || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !sigma || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   CHK(upload_sigma(c, chain, sigma));
   HIPC(hipMemcpy(c->seed + chain, &seed, sizeof(int32_t), hipMemcpyHostToDevice));
   c->have_fields = true;
@@ -523,7 +685,8 @@ extern "C" int z2_set_sigma(z2_ctx *c, int32_t chain, const double *sigma, int32
 }
 
 extern "C" int z2_get_sigma(z2_ctx *c, int32_t chain, double *sigma) {
-  if (!c || !sigma || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !sigma || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   std::vector<int8_t> s((size_t)c->T * c->N2);
   HIPC(hipMemcpyAsync(s.data(), c->sig + (size_t)chain * s.size(), s.size(), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
@@ -531,15 +694,32 @@ extern "C" int z2_get_sigma(z2_ctx *c, int32_t chain, double *sigma) {
   return Z2_OK;
 }
 
+extern "C" int z2_get_lambda(z2_ctx *c, int32_t chain, double *lambda) {
+  if (!c || !lambda || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  const double *l = c->lam.data() + (size_t)c->phys[chain] * c->N;
+  std::copy(l, l + c->N, lambda);
+  return Z2_OK;
+}
+
+extern "C" int z2_set_lambda(z2_ctx *c, int32_t chain, const double *lambda) {
+  if (!c || !lambda || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  for (int i = 0; i < c->N; ++i)
+    if (lambda[i] != 1.0 && lambda[i] != -1.0) return set_err(Z2_EINVAL, "lambda values must be +-1");
+  std::copy(lambda, lambda + c->N, c->lam.data() + (size_t)c->phys[chain] * c->N);
+  return Z2_OK;
+}
+
 extern "C" int z2_get_seed(z2_ctx *c, int32_t chain, int32_t *seed) {
-  if (!c || !seed || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !seed || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   HIPC(hipMemcpyAsync(seed, c->seed + chain, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   return Z2_OK;
 }
 
 static int get_mat(z2_ctx *c, const double *base, int chain, double *out) {
-  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !out || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   size_t nn = (size_t)c->N * c->N;
   HIPC(hipMemcpyAsync(out, base + chain * nn, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
@@ -548,7 +728,8 @@ static int get_mat(z2_ctx *c, const double *base, int chain, double *out) {
 
 extern "C" int z2_get_green(z2_ctx *c, int32_t chain, double *G) {
   if (c && c->mixed) {
-    if (!G || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+    if (!G || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+    chain = c->phys[chain];  // storage slot
     size_t nn = (size_t)c->N * c->N;
     std::vector<float> f(nn);
     HIPC(hipMemcpyAsync(f.data(), c->Gf + chain * nn, sizeof(float) * nn, hipMemcpyDeviceToHost, c->stream));
@@ -560,7 +741,8 @@ extern "C" int z2_get_green(z2_ctx *c, int32_t chain, double *G) {
 }
 
 extern "C" int z2_set_green(z2_ctx *c, int32_t chain, const double *G) {
-  if (!c || !G || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !G || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   size_t nn = (size_t)c->N * c->N;
   if (c->mixed) {
     std::vector<float> f(G, G + nn);
@@ -573,7 +755,8 @@ extern "C" int z2_set_green(z2_ctx *c, int32_t chain, const double *G) {
 }
 
 extern "C" int z2_get_drift(z2_ctx *c, int32_t chain, double *xmax, double *xmean) {
-  if (!c || !xmax || !xmean || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !xmax || !xmean || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   HIPC(hipMemcpyAsync(xmax, c->xmax + chain, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipMemcpyAsync(xmean, c->xmean + chain, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
@@ -581,14 +764,16 @@ extern "C" int z2_get_drift(z2_ctx *c, int32_t chain, double *xmax, double *xmea
 }
 
 extern "C" int z2_get_obs_equal(z2_ctx *c, int32_t chain, double out[7]) {
-  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !out || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   HIPC(hipMemcpyAsync(out, c->obs_res + (size_t)chain * RES_N, sizeof(double) * RES_N, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   return Z2_OK;
 }
 
 extern "C" int z2_get_counters(z2_ctx *c, int32_t chain, uint64_t out[2]) {
-  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !out || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   unsigned long long v[2];
   HIPC(hipMemcpyAsync(v, c->accc + 2 * chain, sizeof(v), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
@@ -714,6 +899,9 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
   f.record = c->rec;
   f.dwords = c->dwords;
   f.lat = c->lat;
+  f.gbond = c->gmode ? c->gbond : nullptr;
+  f.gmask = c->gmode ? c->gmask : nullptr;
+  f.glr = c->gmode ? c->glr : nullptr;
   size_t sh = flip_lds_bytes(c->N, c->kdelay);
   static_assert(FLIP_THREADS == 512, "k_flip is compiled for 512 threads per block");
   const int mpt = (c->N2 + FLIP_THREADS - 1) / FLIP_THREADS;
@@ -1054,8 +1242,8 @@ static int drift_replace(z2_ctx *c, int accumulate) {
   return Z2_OK;
 }
 
-static inline size_t slot(const z2_ctx *c, int st) { return (size_t)st * c->nch * c->N * c->N; }
-static inline size_t slotd(const z2_ctx *c, int st) { return (size_t)st * c->nch * c->N; }
+static inline size_t slot(const z2_ctx *c, int st) { return (size_t)st * c->nch_all * c->N * c->N; }
+static inline size_t slotd(const z2_ctx *c, int st) { return (size_t)st * c->nch_all * c->N; }
 
 extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect preortho time slice %d", nt);
@@ -1207,7 +1395,7 @@ static int obs_ave(z2_ctx *c) {
 extern "C" int z2_enable_tau(z2_ctx *c) {
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (c->tau_on) return Z2_OK;
-  const size_t nn = (size_t)c->N * c->N, B = (size_t)c->nch, n2n2 = 4 * nn;
+  const size_t nn = (size_t)c->N * c->N, B = (size_t)c->nch_all, n2n2 = 4 * nn;
   double **mats[] = {&c->dUUR, &c->dVUR, &c->dUUL, &c->dVUL, &c->G00, &c->G0t, &c->Gt0, &c->Gtt, &c->iVR, &c->iVL};
   for (double **m : mats) CHK(dalloc(c, m, B * nn));
   for (double **m : {&c->bigA, &c->bigQ, &c->bigZ, &c->bigR}) CHK(dalloc(c, m, B * n2n2));
@@ -1354,7 +1542,8 @@ extern "C" int z2_dyn_measure(z2_ctx *c) {
 }
 
 extern "C" int z2_get_obs_tau(z2_ctx *c, int32_t chain, double *out) {
-  if (!c || !out || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !out || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
   const size_t n = (size_t)TAU_NOBS * c->T * c->N;
   HIPC(hipMemcpyAsync(out, c->otau + chain * n, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
@@ -1363,7 +1552,8 @@ extern "C" int z2_get_obs_tau(z2_ctx *c, int32_t chain, double *out) {
 }
 
 extern "C" int z2_get_dyn_drift(z2_ctx *c, int32_t chain, double xmax[3], double xmean[3]) {
-  if (!c || !xmax || !xmean || chain < 0 || chain >= c->nch) return set_err(Z2_EINVAL, "bad argument");
+  if (!c || !xmax || !xmean || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];  // storage slot
   if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
   double v[6];
   HIPC(hipMemcpyAsync(v, c->dyn_drift + 6 * chain, sizeof(v), hipMemcpyDeviceToHost, c->stream));
@@ -1378,8 +1568,8 @@ extern "C" int z2_get_dyn_drift(z2_ctx *c, int32_t chain, double xmax[3], double
 // ---------------------------------------------------------------- drivers
 // Local_sweep_reset's h ~ 0 synchronisation (src/local_sweep.f90:63-69)
 static int h_zero_sync(z2_ctx *c) {
-  for (int b = 0; b < c->nch; ++b) {
-    if (fabs(c->desc[b].h) > 1.0e-10) continue;
+  for (int b = 0; b < c->nch_all; ++b) {
+    if (fabs(c->desc[c->phys[b]].h) > 1.0e-10) continue;
     std::vector<double> s((size_t)c->T * c->N2);
     CHK(z2_get_sigma(c, b, s.data()));
     for (int nt = 1; nt < c->T; ++nt)
@@ -1401,6 +1591,7 @@ extern "C" int z2_pre(z2_ctx *c) {
     CHK(pre_prop(c, nt));
     if (nt % c->w == 0) CHK(z2_wrap_pre(c, nt));
   }
+  c->isb.assign(c->nch_all, 1);  // the reference's main starts every chain with is_beta = .true.
   c->have_pre = true;
   return Z2_OK;
 }
@@ -1411,6 +1602,7 @@ enum StepKind { ST_WRAP_L, ST_WRAP_R, ST_OBS, ST_SLICE_L, ST_SLICE_R, ST_DYN };
 struct Step {
   StepKind kind;
   int nt;
+  int flag = 1;  // wrap steps: 1 adds the drift to its mean (Wrap_L/R flag)
 };
 
 static void steps_L(const z2_ctx *c, std::vector<Step> &v) {
@@ -1446,8 +1638,8 @@ static void sweep_steps(const z2_ctx *c, int is_beta, int toggle, std::vector<St
 
 static int run_step(z2_ctx *c, const Step &st) {
   switch (st.kind) {
-    case ST_WRAP_L: return z2_wrap_L(c, st.nt, 1);
-    case ST_WRAP_R: return z2_wrap_R(c, st.nt, 1);
+    case ST_WRAP_L: return z2_wrap_L(c, st.nt, st.flag);
+    case ST_WRAP_R: return z2_wrap_R(c, st.nt, st.flag);
     case ST_OBS: return z2_obs_equal_calc(c, st.nt);
     case ST_SLICE_L: return z2_slice_L(c, st.nt);
     case ST_SLICE_R: return z2_slice_R(c, st.nt);
@@ -1464,20 +1656,118 @@ static bool step_recomputes(const z2_ctx *c, const Step &st) {
 // Local_sweep with toggle (src/local_sweep.f90:73-150): with toggle the
 // time-displaced observables are reset, measured at the start of every right
 // pass and divided by the number of such passes.
+// ---------------------------------------------------------------- chain order
+// Chains are stored grouped by pass direction so that each group is a
+// contiguous range the batched kernels can run on.
+static int copy_chains(z2_ctx *c, const char *src, char *dst, const ChainField &f, int c0, int cnt,
+                       const int *perm, const int *skip) {
+  if (cnt <= 0) return Z2_OK;
+  const size_t off = (size_t)c0 * f.bytes, sstride = (size_t)c->nch_all * f.bytes;
+  const size_t blocks = (f.bytes / 4 + 255) / 256;
+  dim3 grid((unsigned)(blocks > 256 ? 256 : (blocks ? blocks : 1)), cnt, f.slots);
+  if (f.bytes % 16 == 0)
+    hipLaunchKernelGGL(k_copy_chains<uint4>, grid, dim3(256), 0, c->stream, (const uint4 *)(src + off),
+                       (uint4 *)(dst + off), f.bytes / 16, sstride / 16, perm, skip);
+  else if (f.bytes % 8 == 0)
+    hipLaunchKernelGGL(k_copy_chains<uint2>, grid, dim3(256), 0, c->stream, (const uint2 *)(src + off),
+                       (uint2 *)(dst + off), f.bytes / 8, sstride / 8, perm, skip);
+  else if (f.bytes % 4 == 0)
+    hipLaunchKernelGGL(k_copy_chains<uint32_t>, grid, dim3(256), 0, c->stream, (const uint32_t *)(src + off),
+                       (uint32_t *)(dst + off), f.bytes / 4, sstride / 4, perm, skip);
+  else
+    return set_err(Z2_EINVAL, "per-chain array size %zu is not a multiple of 4 bytes", f.bytes);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+// storage slot j <- slot perm[j] for every array that moves with its chain
+static int reorder(z2_ctx *c, const std::vector<int> &perm) {
+  const int B = c->nch_all;
+  size_t need = 0;
+  for (auto &f : c->fields)  // including arrays not allocated yet, so that the scratch fits them later
+    if (f.flags & CF_PERSIST) need = std::max(need, f.bytes * f.slots * (size_t)B);
+  if (!c->reorder_tmp) CHK(dalloc(c, &c->reorder_tmp, need));
+  HIPC(hipMemcpyAsync(c->d_perm, perm.data(), sizeof(int) * B, hipMemcpyHostToDevice, c->stream));
+  for (auto &f : c->fields) {
+    char *a = f.get();
+    if (!(f.flags & CF_PERSIST) || !a) continue;
+    CHK(copy_chains(c, a, c->reorder_tmp, f, 0, B, c->d_perm, nullptr));
+    HIPC(hipMemcpyAsync(a, c->reorder_tmp, f.bytes * f.slots * (size_t)B, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPC(hipStreamSynchronize(c->stream));  // perm's host buffer is released on return
+  std::vector<int> inv(B);
+  for (int j = 0; j < B; ++j) inv[perm[j]] = j;
+  std::vector<ChainConst> hcc(B);
+  std::vector<z2_chain_desc> desc(B);
+  std::vector<int> isb(B);
+  std::vector<double> lam(c->lam.size());
+  for (int j = 0; j < B; ++j) {
+    hcc[j] = c->hcc[perm[j]];
+    desc[j] = c->desc[perm[j]];
+    isb[j] = c->isb[perm[j]];
+    std::copy(c->lam.begin() + (size_t)perm[j] * c->N, c->lam.begin() + (size_t)(perm[j] + 1) * c->N,
+              lam.begin() + (size_t)j * c->N);
+  }
+  c->hcc.swap(hcc);
+  c->desc.swap(desc);
+  c->isb.swap(isb);
+  c->lam.swap(lam);
+  for (int k = 0; k < B; ++k) c->phys[k] = inv[c->phys[k]];
+  return Z2_OK;
+}
+
+// stores the chains with is_beta = 1 first; *n1 = their number
+static int partition(z2_ctx *c, int *n1) {
+  std::vector<int> perm;
+  for (int j = 0; j < c->nch_all; ++j)
+    if (c->isb[j]) perm.push_back(j);
+  *n1 = (int)perm.size();
+  for (int j = 0; j < c->nch_all; ++j)
+    if (!c->isb[j]) perm.push_back(j);
+  bool ident = true;
+  for (int j = 0; j < c->nch_all; ++j) ident = ident && perm[j] == j;
+  return ident ? Z2_OK : reorder(c, perm);
+}
+
+// the steps of a sweep for the chain range of one direction
+static int run_steps(z2_ctx *c, const std::vector<Step> &v, int *nobst) {
+  for (const Step &st : v) {
+    CHK(run_step(c, st));
+    if (nobst) *nobst += st.kind == ST_DYN;
+  }
+  return Z2_OK;
+}
+
 extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_sweep");
   if (toggle && !c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
+  if (is_beta != 0 && is_beta != 1 && is_beta != Z2_BETA_PER_CHAIN)
+    return set_err(Z2_EINVAL, "is_beta must be 0, 1 or Z2_BETA_PER_CHAIN");
   CHK(h_zero_sync(c));
   CHK(z2_obs_reset(c));
   const size_t ntau = (size_t)c->nch * TAU_NOBS * c->T * c->N;
   if (toggle) HIPC(hipMemsetAsync(c->otau, 0, sizeof(double) * ntau, c->stream));
-  std::vector<Step> v;
-  sweep_steps(c, is_beta, toggle, v);
   int nobst = 0;
-  for (const Step &st : v) {
-    CHK(run_step(c, st));
-    nobst += st.kind == ST_DYN;
+  if (is_beta != Z2_BETA_PER_CHAIN) {
+    c->isb.assign(c->nch_all, is_beta);
+    std::vector<Step> v;
+    sweep_steps(c, is_beta, toggle, v);
+    CHK(run_steps(c, v, &nobst));
+  } else {
+    // each chain in its own order (after Wolff updates the directions differ)
+    int n1;
+    CHK(partition(c, &n1));
+    int nob[2] = {0, 0};
+    for (int g = 0; g < 2; ++g) {
+      const int c0 = g == 0 ? 0 : n1, cnt = g == 0 ? n1 : c->nch_all - n1;
+      if (!cnt) continue;
+      std::vector<Step> v;
+      sweep_steps(c, g == 0 ? 1 : 0, toggle, v);
+      ChainRange r(c, c0, cnt, c->stream);
+      CHK(run_steps(c, v, &nob[g]));
+    }
+    nobst = nob[0] > nob[1] ? nob[0] : nob[1];
   }
   CHK(obs_ave(c));
   if (toggle && nobst > 0) {
@@ -1488,6 +1778,161 @@ extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
 }
 
 extern "C" int z2_sweep(z2_ctx *c, int32_t is_beta) { return z2_sweep_tau(c, is_beta, 0); }
+
+extern "C" int z2_set_is_beta(z2_ctx *c, int32_t chain, int32_t is_beta) {
+  if (!c || chain < 0 || chain >= c->nch_all || (is_beta != 0 && is_beta != 1))
+    return set_err(Z2_EINVAL, "bad argument");
+  c->isb[c->phys[chain]] = is_beta;
+  return Z2_OK;
+}
+
+extern "C" int z2_get_is_beta(z2_ctx *c, int32_t chain, int32_t *is_beta) {
+  if (!c || !is_beta || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  *is_beta = c->isb[c->phys[chain]];
+  return Z2_OK;
+}
+
+// ---------------------------------------------------------------- Wolff global update
+static GlobArgs glob_args(z2_ctx *c) {
+  GlobArgs g;
+  g.sig = c->sig;
+  g.seed = c->seed;
+  g.cc = c->dcc;
+  g.lat = c->lat;
+  g.T = c->T;
+  g.nch = c->nch;
+  g.bond = c->gbond;
+  g.mask = c->gmask;
+  g.queue = c->gqueue;
+  g.lr = c->glr;
+  g.acc = c->gacc;
+  g.cnt = c->gcnt;
+  g.sz = c->gsz;
+  return g;
+}
+
+// copy of everything a trial pass changes (restore = 0), or its restoration for
+// the chains of the range whose trial was rejected (restore = 1)
+static int trial_state(z2_ctx *c, int c0, int cnt, int restore) {
+  if (!c->gbackup) {
+    for (auto &f : c->fields)
+      if ((f.flags & CF_TRIAL) && f.get()) CHK(dalloc(c, &f.backup, f.bytes * f.slots * (size_t)c->nch_all));
+    c->gbackup = true;
+  }
+  for (auto &f : c->fields) {
+    char *a = f.get();
+    if (!(f.flags & CF_TRIAL) || !a) continue;
+    if (restore) {
+      // skip flag = accepted
+      CHK(copy_chains(c, f.backup, a, f, c0, cnt, nullptr, c->gacc + c0));
+    } else {
+      CHK(copy_chains(c, a, f.backup, f, c0, cnt, nullptr, nullptr));
+    }
+  }
+  return Z2_OK;
+}
+
+// Global_sweep_L / Global_sweep_R (src/global_update.f90:229-311) as steps: the
+// propagation of a left or right pass with the flips of the cluster forced and
+// the checkpoints stored without adding to the drift mean (Wrap_L/R flag 0)
+static void global_steps(const z2_ctx *c, int is_beta, std::vector<Step> &v) {
+  if (is_beta) {
+    for (int nt = c->T; nt >= 1; --nt) {
+      if (nt % c->w == 0) v.push_back({ST_WRAP_L, nt, 0});
+      v.push_back({ST_SLICE_L, nt, 0});
+    }
+    v.push_back({ST_WRAP_L, 0, 0});
+  } else {
+    v.push_back({ST_WRAP_R, 0, 0});
+    for (int nt = 1; nt <= c->T; ++nt) {
+      v.push_back({ST_SLICE_R, nt, 0});
+      if (nt % c->w == 0) v.push_back({ST_WRAP_R, nt, 0});
+    }
+  }
+}
+
+// Global_sweep (src/global_update.f90:313-337) for every chain, each in its own
+// direction; an accepted trial reverses the chain's is_beta.
+extern "C" int z2_global_sweep(z2_ctx *c, int32_t nglobal) {
+  if (!c || nglobal < 1) return set_err(Z2_EINVAL, "null ctx or Nglobal < 1");
+  if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_global_sweep");
+  if (c->force) return set_err(Z2_ESTATE, "global sweep during a replay");
+  const dim3 g1((c->nch_all + 63) / 64), b1(64);
+  for (int ngl = 0; ngl < nglobal; ++ngl) {
+    int n1;
+    CHK(partition(c, &n1));
+    for (int g = 0; g < 2; ++g) {
+      const int c0 = g == 0 ? 0 : n1, cnt = g == 0 ? n1 : c->nch_all - n1;
+      if (!cnt) continue;
+      CHK(trial_state(c, c0, cnt, 0));
+      {
+        ChainRange r(c, c0, cnt, c->stream);
+        const dim3 gr((cnt + 63) / 64);
+        hipLaunchKernelGGL(k_wolff_cluster, gr, b1, 0, c->stream, glob_args(c));
+        LAUNCH_CHECK();
+        std::vector<Step> v;
+        global_steps(c, g == 0 ? 1 : 0, v);
+        c->gmode = true;
+        int rc = run_steps(c, v, nullptr);
+        c->gmode = false;
+        CHK(rc);
+        hipLaunchKernelGGL(k_wolff_accept, gr, b1, 0, c->stream, glob_args(c));
+        LAUNCH_CHECK();
+      }
+      CHK(trial_state(c, c0, cnt, 1));
+    }
+    std::vector<int> acc(c->nch_all);
+    HIPC(hipMemcpyAsync(acc.data(), c->gacc, sizeof(int) * c->nch_all, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    for (int j = 0; j < c->nch_all; ++j)
+      if (acc[j]) c->isb[j] = !c->isb[j];
+  }
+  hipLaunchKernelGGL(k_wolff_finish, g1, b1, 0, c->stream, glob_args(c), (int)nglobal);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+extern "C" int z2_get_global_counters(z2_ctx *c, int32_t chain, uint64_t out[3], double *size_cluster) {
+  if (!c || !out || !size_cluster || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];
+  unsigned long long v[3];
+  double z[2];
+  HIPC(hipMemcpyAsync(v, c->gcnt + 3 * chain, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(z, c->gsz + 2 * chain, sizeof(z), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < 3; ++k) out[k] = v[k];
+  *size_cluster = z[1];
+  return Z2_OK;
+}
+
+// LocalK_therm sweep (src/localK.f90:400-420), the boson-only warm-up
+extern "C" int z2_therm(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (!c->have_fields) return set_err(Z2_ESTATE, "fields not initialised (z2_init_fields / z2_set_sigma)");
+  CHK(h_zero_sync(c));  // Local_sweep_reset (src/local_sweep.f90:63-69)
+  ThermArgs a;
+  a.sig = c->sig;
+  a.seed = c->seed;
+  a.cc = c->dcc;
+  a.lat = c->lat;
+  a.T = c->T;
+  a.cnt = c->tcnt;
+  const size_t sh = ((3 * (size_t)c->N2 + 15) & ~(size_t)15) + 16 * sizeof(double);
+  hipLaunchKernelGGL(k_therm, dim3(c->nch_all), dim3(256), sh, c->stream, a);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+extern "C" int z2_get_therm_counters(z2_ctx *c, int32_t chain, uint64_t out[2]) {
+  if (!c || !out || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
+  chain = c->phys[chain];
+  unsigned long long v[2];
+  HIPC(hipMemcpyAsync(v, c->tcnt + 2 * chain, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  out[0] = v[0];
+  out[1] = v[1];
+  return Z2_OK;
+}
 
 template <typename Rb>
 static int maxdiff(z2_ctx *ref, z2_ctx *c, const Rb *Gc) {
@@ -1503,6 +1948,8 @@ extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *
       c->p.Nsweep != ref->p.Nsweep)
     return set_err(Z2_EINVAL, "shadow needs an FP64 reference context of the same shape");
   if (!c->have_pre || !ref->have_pre) return set_err(Z2_ESTATE, "z2_pre must run on both contexts");
+  if (c->phys != ref->phys) return set_err(Z2_ESTATE, "shadow needs both contexts' chains in the same storage order");
+  if (is_beta != 0 && is_beta != 1) return set_err(Z2_EINVAL, "shadow sweeps take is_beta 0 or 1");
   CHK(h_zero_sync(c));
   CHK(h_zero_sync(ref));
   CHK(z2_obs_reset(c));

@@ -9,7 +9,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import Z2ChainDesc, Z2Params, Z2_PREC_FP64, check, lib
+from ._lib import Z2ChainDesc, Z2Params, Z2_BETA_PER_CHAIN, Z2_PREC_FP64, check, lib
 
 OBS_NAMES = ("flux_avg", "chi_flux", "pair_sc_q0", "cdw_pi", "cdw_shift", "cdw_ratio", "density")
 TAU_NAMES = ("spin", "pair_s", "pair_d", "charge_s", "charge_d", "single", "curxx")
@@ -73,12 +73,47 @@ class Engine:
     def pre(self):
         check(lib().z2_pre(self._ptr))
 
-    def sweep(self, is_beta: bool = True, toggle: bool = False):
-        """LocalSweep%sweep; toggle adds the time-displaced measurement (needs enable_tau)."""
+    def sweep(self, is_beta=True, toggle: bool = False):
+        """LocalSweep%sweep; toggle adds the time-displaced measurement (needs enable_tau).
+        is_beta=None runs every chain in its own direction (after global updates)."""
+        b = Z2_BETA_PER_CHAIN if is_beta is None else int(bool(is_beta))
         if toggle:
-            check(lib().z2_sweep_tau(self._ptr, int(is_beta), 1))
+            check(lib().z2_sweep_tau(self._ptr, b, 1))
         else:
-            check(lib().z2_sweep(self._ptr, int(is_beta)))
+            check(lib().z2_sweep(self._ptr, b))
+
+    def global_sweep(self, nglobal: int):
+        """Global_sweep: Nglobal Wolff trials per chain in the chain's own direction."""
+        check(lib().z2_global_sweep(self._ptr, int(nglobal)))
+
+    def therm(self):
+        """one LocalK_therm (boson-only) sweep"""
+        check(lib().z2_therm(self._ptr))
+
+    def lam(self, chain: int) -> np.ndarray:
+        out = np.zeros(self.N)
+        check(lib().z2_get_lambda(self._ptr, chain, _pd(out)))
+        return out
+
+    def is_beta(self, chain: int) -> bool:
+        v = C.c_int32()
+        check(lib().z2_get_is_beta(self._ptr, chain, C.byref(v)))
+        return bool(v.value)
+
+    def set_is_beta(self, chain: int, v: bool):
+        check(lib().z2_set_is_beta(self._ptr, chain, int(bool(v))))
+
+    def global_counters(self, chain: int):
+        """(accepted, proposed, sum of cluster sizes), wolff_size_cluster"""
+        o = (C.c_uint64 * 3)()
+        z = C.c_double()
+        check(lib().z2_get_global_counters(self._ptr, chain, o, C.byref(z)))
+        return (int(o[0]), int(o[1]), int(o[2])), z.value
+
+    def therm_counters(self, chain: int):
+        o = (C.c_uint64 * 2)()
+        check(lib().z2_get_therm_counters(self._ptr, chain, o))
+        return int(o[0]), int(o[1])
 
     def enable_tau(self):
         check(lib().z2_enable_tau(self._ptr))
