@@ -71,91 +71,113 @@ __device__ __forceinline__ R kin_prologue(R x, int r, int col, const KinMat<R> &
   return x;
 }
 
+// Rotation table of one slice in LDS: for each of the 4 checkerboard groups and
+// each of its N/2 bonds, the two sites and the sign of nflag * sigma packed in
+// one int (sites < 2048), so a rotation needs one LDS read instead of a chain of
+// dependent global reads.
+__device__ __forceinline__ int kin_tab_entry(const LatticeDev &lat, const int8_t *s, int g, int k, int nflag) {
+  const int half = lat.N / 2;
+  const int b = lat.grp[g * half + k];
+  const int p1 = lat.bond_p[2 * b], p2 = lat.bond_p[2 * b + 1];
+  return p1 | (p2 << 11) | ((nflag * (int)s[b] > 0) ? (1 << 22) : 0);
+}
+
 // Mat <- B^{nflag} Mat : each column is independent; a block owns CB columns.
+// A thread owns one bond of a group and a set of the block's columns: its
+// rotation data is read once per stage.
 // grid: (ceil(N/CB), n_chains, n_mats)
 template <typename R, int CB>
 __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
-  R *tile = reinterpret_cast<R *>(smem_raw);  // [CB][N]
-  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2;
+  R *tile = reinterpret_cast<R *>(smem_raw);                   // [CB][N]
+  int *tab = reinterpret_cast<int *>(tile + (size_t)CB * a.lat.N);  // [4][N/2]
+  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
   const int chain = blockIdx.y;
   const KinMat<R> km = a.m[blockIdx.z];
   R *M = km.mat + (size_t)chain * N * N;
   const int col0 = blockIdx.x * CB;
   const int ncol = min(CB, N - col0);
   const ChainConst c = a.cc[chain];
-  for (int idx = threadIdx.x; idx < ncol * N; idx += blockDim.x) {
-    int cc = idx / N, r = idx - cc * N;
-    R x = M[r + (size_t)(col0 + cc) * N];
-    tile[cc * N + r] = kin_prologue<R>(x, r, col0 + cc, km, a, c, chain);
-  }
-  __syncthreads();
   if (km.nflag != 0) {
     const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
+    for (int g = 0; g < 4; ++g)
+      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, s, g, k, km.nflag);
+  }
+  for (int cc = 0; cc < ncol; ++cc)
+    for (int r = tid; r < N; r += nt) tile[cc * N + r] = kin_prologue<R>(M[r + (size_t)(col0 + cc) * N], r, col0 + cc, km, a, c, chain);
+  __syncthreads();
+  if (km.nflag != 0) {
+    const R Cp = (R)c.Cp, Cm = (R)c.Cm, Sp = (R)c.Sp, Sm = (R)c.Sm;
+    const int ngrp = half >= nt ? 1 : nt / half;  // column groups
+    const int items = half * ngrp;
     for (int st = 0; st < 8; ++st) {
-      const int *grp = a.lat.grp + kin_group_of_stage(st) * half;
-      for (int idx = threadIdx.x; idx < half * ncol; idx += blockDim.x) {
-        int k = idx % half, cc = idx / half;
-        int b = grp[k];
-        int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
-        int sg = km.nflag * (int)s[b];
-        R C = (R)(sg > 0 ? c.Cp : c.Cm), S = (R)(sg > 0 ? c.Sp : c.Sm);
-        R x1 = tile[cc * N + p1], x2 = tile[cc * N + p2];
-        R v1 = C * x1 + S * x2;
-        R v2 = S * x1 + C * x2;
-        tile[cc * N + p1] = v1;
-        tile[cc * N + p2] = v2;
+      const int *t = tab + kin_group_of_stage(st) * half;
+      for (int it = tid; it < items; it += nt) {
+        const int k = it % half, cg = it / half;
+        const int e = t[k];
+        const int p1 = e & 2047, p2 = (e >> 11) & 2047;
+        const R C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
+        for (int cc = cg; cc < ncol; cc += ngrp) {
+          R x1 = tile[cc * N + p1], x2 = tile[cc * N + p2];
+          R v1 = C * x1 + S * x2;
+          R v2 = S * x1 + C * x2;
+          tile[cc * N + p1] = v1;
+          tile[cc * N + p2] = v2;
+        }
       }
       __syncthreads();
     }
   }
-  for (int idx = threadIdx.x; idx < ncol * N; idx += blockDim.x) {
-    int cc = idx / N, r = idx - cc * N;
-    M[r + (size_t)(col0 + cc) * N] = apply_fac<R>(apply_fac<R>(tile[cc * N + r], km.post, c), km.post2, c);
-  }
+  for (int cc = 0; cc < ncol; ++cc)
+    for (int r = tid; r < N; r += nt)
+      M[r + (size_t)(col0 + cc) * N] = apply_fac<R>(apply_fac<R>(tile[cc * N + r], km.post, c), km.post2, c);
 }
 
-// Mat <- Mat B^{nflag} : each row is independent; a block owns RB rows.
+// Mat <- Mat B^{nflag} : each row is independent; a block owns RB rows.  Thread
+// tid works on row tid % RB of the block and on bonds tid / RB + j * (nt / RB).
 template <typename R, int RB>
 __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
-  R *tile = reinterpret_cast<R *>(smem_raw);  // [N][RB]
-  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2;
+  R *tile = reinterpret_cast<R *>(smem_raw);                   // [N][RB]
+  int *tab = reinterpret_cast<int *>(tile + (size_t)RB * a.lat.N);  // [4][N/2]
+  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
   const int chain = blockIdx.y;
   const KinMat<R> km = a.m[blockIdx.z];
   R *M = km.mat + (size_t)chain * N * N;
   const int row0 = blockIdx.x * RB;
   const int nrow = min(RB, N - row0);
   const ChainConst c = a.cc[chain];
-  for (int idx = threadIdx.x; idx < nrow * N; idx += blockDim.x) {
-    int col = idx / nrow, r = idx - col * nrow;
-    R x = M[(row0 + r) + (size_t)col * N];
-    tile[col * RB + r] = kin_prologue<R>(x, row0 + r, col, km, a, c, chain);
-  }
-  __syncthreads();
+  const int r = tid % RB, cstep = nt / RB;
   if (km.nflag != 0) {
     const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
+    for (int g = 0; g < 4; ++g)
+      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, s, g, k, km.nflag);
+  }
+  if (r < nrow)
+    for (int col = tid / RB; col < N; col += cstep)
+      tile[col * RB + r] = kin_prologue<R>(M[(row0 + r) + (size_t)col * N], row0 + r, col, km, a, c, chain);
+  __syncthreads();
+  if (km.nflag != 0) {
+    const R Cp = (R)c.Cp, Cm = (R)c.Cm, Sp = (R)c.Sp, Sm = (R)c.Sm;
     for (int st = 0; st < 8; ++st) {
-      const int *grp = a.lat.grp + kin_group_of_stage(st) * half;
-      for (int idx = threadIdx.x; idx < half * nrow; idx += blockDim.x) {
-        int r = idx % nrow, k = idx / nrow;
-        int b = grp[k];
-        int p1 = a.lat.bond_p[2 * b], p2 = a.lat.bond_p[2 * b + 1];
-        int sg = km.nflag * (int)s[b];
-        R C = (R)(sg > 0 ? c.Cp : c.Cm), S = (R)(sg > 0 ? c.Sp : c.Sm);
-        R x1 = tile[p1 * RB + r], x2 = tile[p2 * RB + r];
-        R u1 = x1 * C + x2 * S;
-        R u2 = x1 * S + x2 * C;
-        tile[p1 * RB + r] = u1;
-        tile[p2 * RB + r] = u2;
-      }
+      const int *t = tab + kin_group_of_stage(st) * half;
+      if (r < nrow)
+        for (int k = tid / RB; k < half; k += cstep) {
+          const int e = t[k];
+          const int p1 = e & 2047, p2 = (e >> 11) & 2047;
+          const R C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
+          R x1 = tile[p1 * RB + r], x2 = tile[p2 * RB + r];
+          R u1 = x1 * C + x2 * S;
+          R u2 = x1 * S + x2 * C;
+          tile[p1 * RB + r] = u1;
+          tile[p2 * RB + r] = u2;
+        }
       __syncthreads();
     }
   }
-  for (int idx = threadIdx.x; idx < nrow * N; idx += blockDim.x) {
-    int col = idx / nrow, r = idx - col * nrow;
-    M[(row0 + r) + (size_t)col * N] = apply_fac<R>(apply_fac<R>(tile[col * RB + r], km.post, c), km.post2, c);
-  }
+  if (r < nrow)
+    for (int col = tid / RB; col < N; col += cstep)
+      M[(row0 + r) + (size_t)col * N] = apply_fac<R>(apply_fac<R>(tile[col * RB + r], km.post, c), km.post2, c);
 }
 
 // Elementwise prologue only (flush and/or mu factors) over whole matrices.

@@ -23,51 +23,100 @@ struct ObsArgs {
   const double *cosT;    // (2Lx-1) x (2Ly-1) table of cos(qshift_x*dx + pi*dy)
   int Lx, Ly;
   LatticeDev lat;
+  double *part;          // per chain part_stride x 3 tile partial sums
+  int part_stride;
 };
 
-// One workgroup per chain.
+// The N x N pair sums (pair_sc_q0, cdw_pi, cdw_shift) of one slice, one 32 x 32
+// tile of (i, j) pairs per workgroup: G(i, j) and G(j, i) of the tile are staged
+// in LDS by coalesced column reads; the tile's three partial sums go to `part`
+// and k_obs_finish adds them in tile order (deterministic).
+#define OBS_TB 32
 template <typename R>
-__global__ void __launch_bounds__(1024) k_obs_equal(ObsArgs<R> a) {
-  const int N = a.lat.N, N2 = a.lat.N2;
-  const int chain = blockIdx.x;
+__global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
+  const int N = a.lat.N;
+  const int chain = blockIdx.y, nt = (N + OBS_TB - 1) / OBS_TB;
+  const int bi = blockIdx.x % nt, bj = blockIdx.x / nt;
+  const int i0 = bi * OBS_TB, j0 = bj * OBS_TB;
   const R *G = a.G + (size_t)chain * N * N;
-  const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
-  extern __shared__ unsigned char smem_raw[];
-  double *nd = reinterpret_cast<double *>(smem_raw);  // N: n_i = 1 - G_ii
+  __shared__ double A[OBS_TB][OBS_TB + 1];  // A[v][u] = G(i0 + u, j0 + v)
+  __shared__ double B[OBS_TB][OBS_TB + 1];  // B[u][v] = G(j0 + v, i0 + u)
+  __shared__ double ni[OBS_TB], nj[OBS_TB];
   __shared__ double red[16];
-  for (int i = threadIdx.x; i < N; i += blockDim.x) nd[i] = 1.0 - (double)G[i + (size_t)i * N];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < OBS_TB * OBS_TB; e += blockDim.x) {
+    const int c = e / OBS_TB, r = e % OBS_TB;  // constant divisor
+    const int ia = i0 + r, ja = j0 + c, ib = j0 + r, jb = i0 + c;
+    A[c][r] = (ia < N && ja < N) ? (double)G[ia + (size_t)ja * N] : 0.0;
+    B[c][r] = (ib < N && jb < N) ? (double)G[ib + (size_t)jb * N] : 0.0;
+  }
+  if (tid < OBS_TB) {
+    const int i = i0 + tid;
+    ni[tid] = i < N ? 1.0 - (double)G[i + (size_t)i * N] : 0.0;
+  } else if (tid < 2 * OBS_TB) {
+    const int j = j0 + tid - OBS_TB;
+    nj[tid - OBS_TB] = j < N ? 1.0 - (double)G[j + (size_t)j * N] : 0.0;
+  }
   __syncthreads();
-  double pair = 0.0, cdw = 0.0, cdws = 0.0, dens = 0.0, flux = 0.0;
+  double pair = 0.0, cdw = 0.0, cdws = 0.0;
   const int W = 2 * a.Lx - 1;
-  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
-    int jj = idx / N, ii = idx - jj * N;  // consecutive threads: consecutive ii (coalesced G(ii,jj))
-    double n_i = nd[ii], n_j = nd[jj];
-    double delta = ii == jj ? 1.0 : 0.0;
-    double gij = (double)G[ii + (size_t)jj * N], gji = (double)G[jj + (size_t)ii * N];
-    double x = delta - gij;
+  for (int e = tid; e < OBS_TB * OBS_TB; e += blockDim.x) {
+    const int v = e / OBS_TB, u = e % OBS_TB;
+    const int ii = i0 + u, jj = j0 + v;
+    if (ii >= N || jj >= N) continue;
+    const double n_i = ni[u], n_j = nj[v];
+    const double delta = ii == jj ? 1.0 : 0.0;
+    const double gij = A[v][u], gji = B[u][v];
+    const double x = delta - gij;
     pair += x * x;
     double dc = n_i * n_j - (delta - gji) * gij;
     dc = dc + n_i * n_j - (delta - gji) * gij;
     dc = dc + n_i * n_j + n_i * n_j;
-    double ph = (double)(a.lat.stag[ii] * a.lat.stag[jj]);
-    int dx = a.lat.xy[2 * ii] - a.lat.xy[2 * jj] + a.Lx - 1;
-    int dy = a.lat.xy[2 * ii + 1] - a.lat.xy[2 * jj + 1] + a.Ly - 1;
+    const double ph = (double)(a.lat.stag[ii] * a.lat.stag[jj]);
+    const int dx = a.lat.xy[2 * ii] - a.lat.xy[2 * jj] + a.Lx - 1;
+    const int dy = a.lat.xy[2 * ii + 1] - a.lat.xy[2 * jj + 1] + a.Ly - 1;
     cdw += ph * dc;
     cdws += a.cosT[dx + W * dy] * dc;
-  }
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    dens += nd[i] + nd[i];
-    const int *pl = a.lat.plaq + 4 * i;
-    flux += (double)((int)s[pl[0]] * (int)s[pl[1]] * (int)s[pl[2]] * (int)s[pl[3]]);
   }
   pair = block_reduce_sum(pair, red);
   cdw = block_reduce_sum(cdw, red);
   cdws = block_reduce_sum(cdws, red);
+  if (tid == 0) {
+    double *p = a.part + ((size_t)chain * a.part_stride + blockIdx.x) * 3;
+    p[0] = pair;
+    p[1] = cdw;
+    p[2] = cdws;
+  }
+}
+
+// density and flux of the slice, the tile partials in tile order, and the
+// accumulation into the bin (one workgroup of 256 threads per chain)
+template <typename R>
+__global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
+  const int N = a.lat.N, N2 = a.lat.N2;
+  const int chain = blockIdx.x, ntile = ((N + OBS_TB - 1) / OBS_TB) * ((N + OBS_TB - 1) / OBS_TB);
+  const R *G = a.G + (size_t)chain * N * N;
+  const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
+  __shared__ double red[16];
+  double dens = 0.0, flux = 0.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    const double nd = 1.0 - (double)G[i + (size_t)i * N];
+    dens += nd + nd;
+    const int *pl = a.lat.plaq + 4 * i;
+    flux += (double)((int)s[pl[0]] * (int)s[pl[1]] * (int)s[pl[2]] * (int)s[pl[3]]);
+  }
   dens = block_reduce_sum(dens, red);
   flux = block_reduce_sum(flux, red);
   if (threadIdx.x == 0) {
+    const double *p = a.part + (size_t)chain * a.part_stride * 3;
+    double pair = 0.0, cdw = 0.0, cdws = 0.0;
+    for (int t = 0; t < ntile; ++t) {
+      pair += p[3 * t];
+      cdw += p[3 * t + 1];
+      cdws += p[3 * t + 2];
+    }
     double *acc = a.acc + (size_t)chain * OBS_NACC;
-    double ft = flux / (double)N;
+    const double ft = flux / (double)N;
     acc[OBS_FLUX_SUM] += ft;
     acc[OBS_FLUX_SQ] += ft * ft;
     acc[OBS_PAIR] += pair;

@@ -102,6 +102,8 @@ struct z2_ctx {
   unsigned long long *accc = nullptr;
   double *xmax = nullptr, *xmean = nullptr;
   double *obs_acc = nullptr, *obs_res = nullptr;
+  double *opart = nullptr;  // per chain: tile partial sums of the equal-time observables
+  int opart_stride = 0;     // tiles per chain
   // scratch
   double *M = nullptr, *Q = nullptr, *X = nullptr, *Y = nullptr, *Gn = nullptr;
   double *tau = nullptr, *Dtmp = nullptr;
@@ -230,6 +232,7 @@ static void register_fields(z2_ctx *c) {
   reg_field(c, c->xmean, 1, 1, PT);
   reg_field(c, c->obs_acc, OBS_NACC, 1, P);
   reg_field(c, c->obs_res, RES_N, 1, P);
+  reg_field(c, c->opart, 3 * (size_t)c->opart_stride, 1, 0);
   for (double **m : {&c->M, &c->Q, &c->X, &c->Y, &c->Gn, &c->qrw}) reg_field(c, *m, nn, 1, 0);
   reg_field(c, c->tau, N, 1, 0);
   reg_field(c, c->Dtmp, N, 1, 0);
@@ -487,6 +490,11 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if ((rc = dalloc(c, &c->xmean, B))) break;
     if ((rc = dalloc(c, &c->obs_acc, B * OBS_NACC))) break;
     if ((rc = dalloc(c, &c->obs_res, B * RES_N))) break;
+    {
+      const int ntb = (N + OBS_TB - 1) / OBS_TB;
+      c->opart_stride = ntb * ntb;
+      if ((rc = dalloc(c, &c->opart, B * 3 * (size_t)c->opart_stride))) break;
+    }
     if ((rc = dalloc(c, &c->M, B * nn))) break;
     if ((rc = dalloc(c, &c->Q, B * nn))) break;
     if ((rc = dalloc(c, &c->X, B * nn))) break;
@@ -847,7 +855,9 @@ static const int KRB = 16;  // rows per block, column-rotation kernel
 template <typename R>
 static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
   dim3 grid((c->N + KCB - 1) / KCB, c->nch, nmat);
-  size_t sh = sizeof(R) * KCB * c->N;
+  size_t sh = sizeof(R) * KCB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+  if (sh > 65536)
+    HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, KCB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   CHK(prof_begin(c, Z2_K_KIN_ROWS));
   hipLaunchKernelGGL((k_kin_rows<R, KCB>), grid, dim3(256), sh, c->stream, a);
   LAUNCH_CHECK();
@@ -858,7 +868,9 @@ static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
 template <typename R>
 static int launch_cols(z2_ctx *c, KinArgs<R> &a, int nmat) {
   dim3 grid((c->N + KRB - 1) / KRB, c->nch, nmat);
-  size_t sh = sizeof(R) * KRB * c->N;
+  size_t sh = sizeof(R) * KRB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+  if (sh > 65536)
+    HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, KRB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   CHK(prof_begin(c, Z2_K_KIN_COLS));
   hipLaunchKernelGGL((k_kin_cols<R, KRB>), grid, dim3(256), sh, c->stream, a);
   LAUNCH_CHECK();
@@ -1365,21 +1377,33 @@ extern "C" int z2_obs_reset(z2_ctx *c) {
   return Z2_OK;
 }
 
+template <typename R>
+static int obs_equal_launch(z2_ctx *c, const R *G, int nt) {
+  ObsArgs<R> o;
+  o.G = G;
+  o.sig = c->sig;
+  o.T = c->T;
+  o.nt = nt;
+  o.acc = c->obs_acc;
+  o.cosT = c->d_cosT;
+  o.Lx = c->Lx;
+  o.Ly = c->Ly;
+  o.lat = c->lat;
+  o.part = c->opart;
+  o.part_stride = c->opart_stride;
+  const int ntb = (c->N + OBS_TB - 1) / OBS_TB;
+  hipLaunchKernelGGL((k_obs_tiles<R>), dim3(ntb * ntb, c->nch), dim3(256), 0, c->stream, o);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL((k_obs_finish<R>), dim3(c->nch), dim3(256), 0, c->stream, o);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
 extern "C" int z2_obs_equal_calc(z2_ctx *c, int32_t nt) {
   if (!c || nt < 1 || nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", nt);
   CHK(prof_begin(c, Z2_K_OBS));
-  if (c->mixed) {
-    ObsArgs<float> o;
-    o.G = c->Gf; o.sig = c->sig; o.T = c->T; o.nt = nt; o.acc = c->obs_acc; o.cosT = c->d_cosT;
-    o.Lx = c->Lx; o.Ly = c->Ly; o.lat = c->lat;
-    hipLaunchKernelGGL((k_obs_equal<float>), dim3(c->nch), dim3(1024), sizeof(double) * c->N, c->stream, o);
-  } else {
-    ObsArgs<double> o;
-    o.G = c->G; o.sig = c->sig; o.T = c->T; o.nt = nt; o.acc = c->obs_acc; o.cosT = c->d_cosT;
-    o.Lx = c->Lx; o.Ly = c->Ly; o.lat = c->lat;
-    hipLaunchKernelGGL((k_obs_equal<double>), dim3(c->nch), dim3(1024), sizeof(double) * c->N, c->stream, o);
-  }
-  LAUNCH_CHECK();
+  if (c->mixed) CHK(obs_equal_launch<float>(c, c->Gf, nt));
+  else CHK(obs_equal_launch<double>(c, c->G, nt));
   CHK(prof_end(c, Z2_K_OBS));
   return Z2_OK;
 }
