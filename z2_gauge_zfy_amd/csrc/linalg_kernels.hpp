@@ -568,6 +568,89 @@ __global__ void __launch_bounds__(256) k_trsm(const double *Rm, double *X, int N
   }
 }
 
+// Same solve with X in registers: a group of 16 lanes owns one column of X
+// (lane l holds rows l + 16 j, j < MJ).  Back substitution by 16-row blocks:
+// the block's rows are solved with the pivot divided by its owner lane and
+// broadcast by a lane shuffle (no barrier), then the rows above take the
+// block's 16 updates.  Every element still receives its updates one by one in
+// descending k with one multiply and one subtract each: the arithmetic and its
+// order are those of k_trsm (and ZTRSM).  Columns of R are staged in LDS 16 at
+// a time.  grid: (ceil(N/16), chains), 256 threads = 16 columns.
+template <int MJ>
+__global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, int N, int rs, int cs, int mode) {
+  extern __shared__ unsigned char smem_raw[];
+  double *rt = reinterpret_cast<double *>(smem_raw);  // [16][N]: columns 16 jk + kk of R
+  const int chain = blockIdx.y;
+  const double *R = Rm + (size_t)chain * N * N;
+  double *Xc = X + (size_t)chain * N * N;
+  const int lane = threadIdx.x & 63, l16 = lane & 15, gbase = lane & ~15;
+  const int c = blockIdx.x * 16 + (threadIdx.x >> 4);
+  const bool cin = c < N;
+  double xr[MJ];
+#pragma unroll
+  for (int j = 0; j < MJ; ++j) {
+    const int i = l16 + 16 * j;
+    xr[j] = (cin && i < N) ? Xc[(size_t)i * rs + (size_t)c * cs] : 0.0;
+  }
+  for (int jk = (N - 1) / 16; jk >= 0; --jk) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      double v[16];  // all 16 reads in flight before any is used (clamped column, zeroed below)
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) v[kk] = R[i + (size_t)min(16 * jk + kk, N - 1) * N];
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) rt[kk * N + i] = 16 * jk + kk < N ? v[kk] : 0.0;
+    }
+    __syncthreads();
+    // the block's rows (slot jk) in one register
+    double xb = 0.0;
+#pragma unroll
+    for (int j = 0; j < MJ; ++j)
+      if (j == jk) xb = xr[j];
+    const int ib = 16 * jk + l16;  // this lane's row in the block
+    double pv[16];
+#pragma unroll
+    for (int kk = 15; kk >= 0; --kk) {
+      const int k = 16 * jk + kk;
+      if (k >= N) {
+        pv[kk] = 0.0;
+        continue;
+      }
+      const double *rk = rt + kk * N;
+      const double rkk = rk[k];
+      const double xo = mode == 0 ? xb / rkk : (1.0 / rkk) * xb;  // meaningful in lane kk
+      if (l16 == kk) xb = xo;
+      const double xk = __shfl(xo, gbase | kk, 64);
+      pv[kk] = xk;
+      if (l16 < kk) {
+        const double r = rk[ib];
+        xb = mode == 0 ? xb - xk * r : xb - r * xk;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < MJ; ++j)
+      if (j == jk) xr[j] = xb;
+    // rows above the block: updates k = 16 jk + 15 .. 16 jk in that order
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+      if (j >= jk) continue;  // uniform
+      const int i = l16 + 16 * j;
+      double x = xr[j];
+#pragma unroll
+      for (int kk = 15; kk >= 0; --kk) {
+        const double r = rt[kk * N + i];
+        x = mode == 0 ? x - pv[kk] * r : x - r * pv[kk];
+      }
+      xr[j] = x;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < MJ; ++j) {
+    const int i = l16 + 16 * j;
+    if (cin && i < N) Xc[(size_t)i * rs + (size_t)c * cs] = xr[j];
+  }
+}
+
 // ------------------------------------------------------------------ helpers
 // out = op(in) with optional permutation / scaling, one element per thread.
 // mode:
@@ -596,8 +679,10 @@ __global__ void __launch_bounds__(256) k_elem(ElemArgs e) {
   double *out = e.out + (size_t)chain * N * N;
   const double *d = e.d ? e.d + (size_t)chain * N : nullptr;
   const int *p = e.p ? e.p + (size_t)chain * N : nullptr;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N; idx += gridDim.x * blockDim.x) {
-    int c = idx / N, r = idx - c * N;
+  // a block walks whole columns (no index division; coalesced in r)
+  for (int c = blockIdx.x; c < N; c += gridDim.x)
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    const size_t idx = r + (size_t)c * N;
     switch (e.mode) {
       case 0: out[idx] = in[idx]; break;
       case 1: out[idx] = in[c + (size_t)r * N]; break;
@@ -620,8 +705,9 @@ __global__ void __launch_bounds__(256) k_build_m(const double *X, const double *
   const double *x = X + (size_t)chain * N * N, *v = VRVL + (size_t)chain * N * N;
   const double *dr = DR + (size_t)chain * N, *dl = DL + (size_t)chain * N;
   double *m = M + (size_t)chain * N * N;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N; idx += gridDim.x * blockDim.x) {
-    int c = idx / N, r = idx - c * N;
+  for (int c = blockIdx.x; c < N; c += gridDim.x)
+  for (int r = threadIdx.x; r < N; r += blockDim.x) {
+    const size_t idx = r + (size_t)c * N;
     double t = 0.0 + dl[c] * v[idx];
     double w = 0.0 + dr[r] * t;
     double val = x[idx] + w;

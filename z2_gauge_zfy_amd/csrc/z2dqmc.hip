@@ -1164,7 +1164,27 @@ static int orgqr(z2_ctx *c, const double *A, double *Qo) { return orgqr_n(c, A, 
 
 // n x n triangular solves for nb matrices; the right-hand-side block shrinks so that
 // (cbt + 1) n doubles fit the LDS of one workgroup
+template <int MJ>
+static int launch_trsm_reg(int n, int nb, hipStream_t st, const double *R, double *X, int right) {
+  const size_t sh = sizeof(double) * 16 * (size_t)n;
+  if (sh > 65536)
+    HIPC(hipFuncSetAttribute((const void *)k_trsm_reg<MJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  dim3 grid((n + 15) / 16, nb);
+  if (right) hipLaunchKernelGGL(k_trsm_reg<MJ>, grid, dim3(256), sh, st, R, X, n, n, 1, 1);
+  else hipLaunchKernelGGL(k_trsm_reg<MJ>, grid, dim3(256), sh, st, R, X, n, 1, n, 0);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
 static int launch_trsm(int n, int nb, hipStream_t st, size_t lds_max, const double *R, double *X, int right) {
+  // register kernel: X column in 16 lanes, up to 64 rows per lane
+  if (sizeof(double) * 16 * (size_t)n <= lds_max) {
+    if (n <= 64) return launch_trsm_reg<4>(n, nb, st, R, X, right);
+    if (n <= 128) return launch_trsm_reg<8>(n, nb, st, R, X, right);
+    if (n <= 256) return launch_trsm_reg<16>(n, nb, st, R, X, right);
+    if (n <= 512) return launch_trsm_reg<32>(n, nb, st, R, X, right);
+    if (n <= 1024) return launch_trsm_reg<64>(n, nb, st, R, X, right);
+  }
   int cbt = CBT;
   while (cbt > 1 && sizeof(double) * (size_t)(cbt + 1) * n > lds_max) --cbt;
   size_t sh = sizeof(double) * (size_t)(cbt + 1) * n;
