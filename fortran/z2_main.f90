@@ -1,7 +1,9 @@
 ! Driver with the structure of the reference program (src/main.f90:1-137):
-! init, boson warm-up (is_warm), pre, Nbin bins of [Wolff global sweep
-! (is_global) + local sweep] with the per-bin equal-time observable files
-! appended (src/fourier_trans.f90:251-278), final configuration written.
+! init (fresh or from confin.txt), boson warm-up (is_warm), pre, Nbin bins of
+! [Wolff global sweep (is_global) + local sweep, with the time-displaced
+! measurement after Nthermal bins when is_tau] with the per-bin equal-time and
+! time-displaced files appended (src/fourier_trans.f90:251-449), final
+! configuration written.
 ! All chains of seeds.txt (or the first N: second argument, the reference's
 ! MPI size) run in this one process on the GPU.
 ! Usage: z2_gauge_gpu.out [fp64|mixed] [N]
@@ -12,6 +14,7 @@ program z2_gauge_gpu
     use Stabilize_mod
     use LocalSweep_mod
     use GlobalUpdate_mod
+    use FourierTrans_mod
     implicit none
     integer :: iseed, nbc, nth
     logical :: is_beta, istau_tmp
@@ -19,6 +22,7 @@ program z2_gauge_gpu
     real(kind=8) :: xmax, xmean
     type(LocalSweep) :: Sweep_local
     type(GlobalUpdate) :: Sweep_global
+    type(FourierTrans) :: Fourier
     type(Propagator), allocatable :: PropU, PropD
     type(WrapList), allocatable :: WrU, WrD
     character(len=32) :: arg
@@ -52,9 +56,11 @@ program z2_gauge_gpu
     call Sweep_local%pre(PropU, PropD, WrU, WrD)
     is_beta = .true.; istau_tmp = .false.
     do nbc = 1, Nbin
+        if (nbc > Nthermal) istau_tmp = is_tau
         if (is_global) call Sweep_global%sweep(PropU, PropD, WrU, WrD, iseed, is_beta)
         call Sweep_local%sweep(PropU, PropD, WrU, WrD, iseed, is_beta, istau_tmp)
         call write_obs_equal()
+        if (istau_tmp) call Fourier%prtau(Obs_tau)
     enddo
     if (is_global) call Sweep_global%ctrl_print()
     call Sweep_local%ctrl_print_l(istau_tmp)

@@ -47,18 +47,12 @@ contains
         integer, allocatable :: seeds(:)
         integer :: k, ios, s, conf0
         call read_input()
-        if (is_tau) then
-            write(6, *) "note: this driver writes no time-displaced output files; is_tau is ignored"
-        endif
         if (NlxTherm /= Nlx .or. NlyTherm /= Nly .or. LtrotTherm /= Ltrot) then
             write(6, *) "thermalisation-size transfer is not supported by this driver"; stop 1
         endif
         open(unit=30, file='confin.txt', status='old')
         read(30, *) conf0
         close(30)
-        if (conf0 /= 0) then
-            write(6, *) "restart from confin.txt is not supported by this driver"; stop 1
-        endif
         allocate(seeds(4096))
         n_chains = 0
         open(unit=10, file='seeds.txt', status='old')
@@ -80,9 +74,58 @@ contains
         p = z2_params(int(Nlx, c_int32_t), int(Nly, c_int32_t), int(Ltrot, c_int32_t), &
                       int(Nwrap, c_int32_t), int(Nsweep, c_int32_t), int(precision_mode, c_int32_t))
         call z2_check(z2_ctx_create(p, int(n_chains, c_int32_t), d, ctx), "z2_ctx_create")
-        call z2_check(z2_init_fields(ctx), "z2_init_fields")
+        if (conf0 == 0) then
+            call z2_check(z2_init_fields(ctx), "z2_init_fields")
+        else
+            call conf_restart()
+        endif
         call z2_check(z2_get_seed(ctx, 0_c_int32_t, iseed), "z2_get_seed")
     end subroutine Model_init
+
+    ! restart from confin.txt (conf_therm_in, src/fields.f90:227-309, at the run size):
+    ! per chain its seed, sigma(ii, nt) with nt outer, then lambda(1:Lq); a missing
+    ! lambda is drawn from the chain's seed (lambda_random_fill, src/fields.f90:559-603)
+    subroutine conf_restart()
+        real(c_double), allocatable :: sig(:), lam(:)
+        integer :: k, i, ios
+        integer(c_int32_t) :: s
+        allocate(sig(2 * Lq * Ltrot), lam(Lq))
+        open(unit=30, file='confin.txt', status='old')
+        do k = 0, n_chains - 1
+            read(30, *) s
+            do i = 1, size(sig)
+                read(30, *) sig(i)
+            enddo
+            ios = 0
+            do i = 1, Lq
+                read(30, *, iostat=ios) lam(i)
+                if (ios /= 0) exit
+            enddo
+            if (ios /= 0) call lambda_fill(lam, s)
+            call z2_check(z2_set_sigma(ctx, int(k, c_int32_t), sig, s), "z2_set_sigma")
+            call z2_check(z2_set_lambda(ctx, int(k, c_int32_t), lam), "z2_set_lambda")
+        enddo
+        close(30)
+    end subroutine conf_restart
+
+    ! lambda_random_fill + lambda_enforce_sector with the ranf stream of the seed
+    subroutine lambda_fill(lam, iq)
+        real(c_double), intent(out) :: lam(:)
+        integer(c_int32_t), intent(inout) :: iq
+        integer(kind=8) :: x
+        real(kind=8) :: prod
+        integer :: i
+        prod = 1.d0
+        x = iand(int(iq, 8), 2147483647_8)
+        do i = 1, size(lam)
+            x = mod(x * 48828125_8, 2147483648_8)
+            lam(i) = -1.d0
+            if (dble(x) / 2147483648.d0 >= 0.5d0) lam(i) = 1.d0
+            prod = prod * lam(i)
+        enddo
+        if ((Q >= 0.d0 .and. prod < 0.d0) .or. (Q < 0.d0 .and. prod > 0.d0)) lam(size(lam)) = -lam(size(lam))
+        iq = int(x, c_int32_t)
+    end subroutine lambda_fill
 
     ! Model_clear -> conf_out (src/model.f90:38-45, src/fields.f90:509-557):
     ! seed, sigma(ii, nt), lambda(i) of every chain
@@ -230,12 +273,291 @@ contains
     end subroutine Obs_equal_ave
 end module ObserEqual_mod
 
+module ObserTau_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod, only: ctx, n_chains, Lq, Ltrot
+    implicit none
+    ! src/obser_tau.f90:7-16: Lq x Ltrot correlators at displacement imj and slice
+    type, public :: ObserTau
+        complex(kind=8), allocatable :: spin_tau(:,:), pair_s_tau(:,:), pair_d_tau(:,:)
+        complex(kind=8), allocatable :: charge_s_tau(:,:), charge_d_tau(:,:), single_tau(:,:), curxx_tau(:,:)
+    contains
+        procedure :: make => Obs_tau_make
+        procedure :: reset => Obs_tau_reset
+        procedure :: ave => Obs_tau_ave
+    end type ObserTau
+contains
+    subroutine Obs_tau_make(this)
+        class(ObserTau), intent(inout) :: this
+        allocate(this%spin_tau(Lq, Ltrot), this%pair_s_tau(Lq, Ltrot), this%pair_d_tau(Lq, Ltrot))
+        allocate(this%charge_s_tau(Lq, Ltrot), this%charge_d_tau(Lq, Ltrot), this%single_tau(Lq, Ltrot))
+        allocate(this%curxx_tau(Lq, Ltrot))
+        call this%reset()
+    end subroutine Obs_tau_make
+    subroutine Obs_tau_reset(this)
+        class(ObserTau), intent(inout) :: this
+        this%spin_tau = (0.d0, 0.d0); this%pair_s_tau = (0.d0, 0.d0); this%pair_d_tau = (0.d0, 0.d0)
+        this%charge_s_tau = (0.d0, 0.d0); this%charge_d_tau = (0.d0, 0.d0); this%single_tau = (0.d0, 0.d0)
+        this%curxx_tau = (0.d0, 0.d0)
+    end subroutine Obs_tau_reset
+    ! Obs_tau%ave (src/obser_tau.f90:37-47, done by the engine per chain) + the rank
+    ! average of m_process_obs_tau (src/fourier_trans.f90:451-488) over the chains
+    subroutine Obs_tau_ave(this, Nobst)
+        class(ObserTau), intent(inout) :: this
+        integer, intent(in) :: Nobst
+        real(c_double), allocatable :: o(:, :, :)
+        integer :: k
+        allocate(o(Lq, Ltrot, 7))
+        call this%reset()
+        do k = 0, n_chains - 1
+            call z2_check(z2_get_obs_tau(ctx, int(k, c_int32_t), o), "z2_get_obs_tau")
+            this%spin_tau = this%spin_tau + o(:, :, 1)
+            this%pair_s_tau = this%pair_s_tau + o(:, :, 2)
+            this%pair_d_tau = this%pair_d_tau + o(:, :, 3)
+            this%charge_s_tau = this%charge_s_tau + o(:, :, 4)
+            this%charge_d_tau = this%charge_d_tau + o(:, :, 5)
+            this%single_tau = this%single_tau + o(:, :, 6)
+            this%curxx_tau = this%curxx_tau + o(:, :, 7)
+        enddo
+        this%spin_tau = this%spin_tau / dcmplx(dble(n_chains), 0.d0)
+        this%pair_s_tau = this%pair_s_tau / dcmplx(dble(n_chains), 0.d0)
+        this%pair_d_tau = this%pair_d_tau / dcmplx(dble(n_chains), 0.d0)
+        this%charge_s_tau = this%charge_s_tau / dcmplx(dble(n_chains), 0.d0)
+        this%charge_d_tau = this%charge_d_tau / dcmplx(dble(n_chains), 0.d0)
+        this%single_tau = this%single_tau / dcmplx(dble(n_chains), 0.d0)
+        this%curxx_tau = this%curxx_tau / dcmplx(dble(n_chains), 0.d0)
+        if (Nobst < 0) continue
+    end subroutine Obs_tau_ave
+end module ObserTau_mod
+
+! Output of the time-displaced correlators (m_write_obs_tau, src/fourier_trans.f90:316-449):
+! naive Fourier transform of the displacement index to the Lq lattice momenta
+! (Fourier_R_to_K, src/lattice.f90:266-290), time integration to susceptibilities,
+! files appended per bin.  Momentum and displacement tables as src/lattice.f90:157-175.
+module FourierTrans_mod
+    use DQMC_Model_mod, only: Nlx, Nly, Lq, Ltrot, Dtau, Beta
+    use ObserTau_mod
+    implicit none
+    real(kind=8), allocatable, private :: xk_v(:,:), aimj_v(:,:), k_dot_r(:,:)
+    integer, allocatable, private :: inv_n_list(:,:)
+    type, public :: FourierTrans
+    contains
+        procedure, nopass :: init => Fourier_init
+        procedure, nopass :: prtau => Fourier_prtau
+    end type FourierTrans
+contains
+    subroutine Fourier_init()
+        real(kind=8) :: pi
+        integer :: ix, iy, ii, nk
+        pi = acos(-1.d0)
+        if (allocated(xk_v)) return
+        allocate(xk_v(Lq, 2), aimj_v(Lq, 2), k_dot_r(Lq, Lq), inv_n_list(Nlx, Nly))
+        do iy = 1, Nly
+            do ix = 1, Nlx
+                ii = (iy - 1) * Nlx + ix
+                inv_n_list(ix, iy) = ii
+                aimj_v(ii, 1) = dble(ix)
+                aimj_v(ii, 2) = dble(iy)
+                xk_v(ii, 1) = dble(ix - 1) * (2.d0 * pi) / dble(Nlx)
+                xk_v(ii, 2) = dble(iy - 1) * (2.d0 * pi) / dble(Nly)
+            enddo
+        enddo
+        do ii = 1, Lq
+            do nk = 1, Lq
+                k_dot_r(nk, ii) = xk_v(nk, 1) * aimj_v(ii, 1) + xk_v(nk, 2) * aimj_v(ii, 2)
+            enddo
+        enddo
+    end subroutine Fourier_init
+
+    ! gk(k, t) = sum over imj of exp(i k.r_imj) gr(imj, t) / Lq, imj outer
+    subroutine r_to_k(gr, gk)
+        complex(kind=8), intent(in) :: gr(:,:)
+        complex(kind=8), intent(out) :: gk(:,:)
+        integer :: nf, imj, nk
+        gk = (0.d0, 0.d0)
+        do nf = 1, size(gr, 2)
+            do imj = 1, Lq
+                do nk = 1, Lq
+                    gk(nk, nf) = gk(nk, nf) + exp(dcmplx(0.d0, k_dot_r(nk, imj))) * gr(imj, nf)
+                enddo
+            enddo
+        enddo
+        gk = gk / dble(Lq)
+    end subroutine r_to_k
+
+    ! Dtau * sum over the slices (m_integrate_susc_2_mom)
+    subroutine integrate_susc(gr, gk)
+        complex(kind=8), intent(in) :: gr(:,:)
+        complex(kind=8), intent(out) :: gk(:)
+        integer :: nt, nk
+        gk = (0.d0, 0.d0)
+        do nt = 1, Ltrot
+            do nk = 1, Lq
+                gk(nk) = gk(nk) + gr(nk, nt)
+            enddo
+        enddo
+        gk = gk * dcmplx(Dtau, 0.d0)
+    end subroutine integrate_susc
+
+    ! Dtau * sum over slices of exp(2 pi i (nt-1)(nw-1)/Ltrot) gr(1, nt) (m_integrate_susc_2_freq)
+    subroutine integrate_susc_freq(gr, gw)
+        complex(kind=8), intent(in) :: gr(:,:)
+        complex(kind=8), intent(out) :: gw(:)
+        integer :: nt, nw
+        real(kind=8) :: pi
+        pi = acos(-1.d0)
+        gw = (0.d0, 0.d0)
+        do nw = 1, Ltrot
+            do nt = 1, Ltrot
+                gw(nw) = gw(nw) + exp(dcmplx(0.d0, 2.d0 * pi * dble((nt - 1) * (nw - 1)) / dble(Ltrot))) * gr(1, nt)
+            enddo
+        enddo
+        gw = gw * dcmplx(Dtau, 0.d0)
+    end subroutine integrate_susc_freq
+
+    subroutine write_reciprocal(gk, filek)
+        complex(kind=8), intent(in) :: gk(:)
+        character(len=*), intent(in) :: filek
+        integer :: nk
+        open(unit=20, file=filek, status='unknown', action='write', position='append')
+        do nk = 1, Lq
+            write(20, *) xk_v(nk, 1), xk_v(nk, 2)
+            write(20, *) gk(nk)
+        enddo
+        close(20)
+    end subroutine write_reciprocal
+
+    subroutine write_k(gk, filek, idx)
+        complex(kind=8), intent(in) :: gk(:)
+        character(len=*), intent(in) :: filek
+        integer, intent(in) :: idx
+        open(unit=30, file=filek, status='unknown', action='write', position='append')
+        write(30, *) real(gk(idx)), aimag(gk(idx))
+        close(30)
+    end subroutine write_k
+
+    subroutine write_k_tau(gk, filek, idx)
+        complex(kind=8), intent(in) :: gk(:,:)
+        character(len=*), intent(in) :: filek
+        integer, intent(in) :: idx
+        integer :: nt
+        open(unit=20, file=filek, status='unknown', action='write', position='append')
+        write(20, *) xk_v(idx, 1), xk_v(idx, 2)
+        do nt = 1, Ltrot
+            write(20, *) real(gk(idx, nt)), aimag(gk(idx, nt))
+        enddo
+        close(20)
+    end subroutine write_k_tau
+
+    subroutine write_r_tau(gr, filek, idx)
+        complex(kind=8), intent(in) :: gr(:,:)
+        character(len=*), intent(in) :: filek
+        integer, intent(in) :: idx
+        integer :: nt
+        complex(kind=8) :: tmp
+        open(unit=20, file=filek, status='unknown', action='write', position='append')
+        write(20, *) aimj_v(idx, 1), aimj_v(idx, 2)
+        do nt = 1, Ltrot
+            tmp = gr(idx, nt) / dble(Lq)
+            write(20, *) real(tmp), aimag(tmp)
+        enddo
+        close(20)
+    end subroutine write_r_tau
+
+    subroutine write_w(gw, filek)
+        complex(kind=8), intent(in) :: gw(:)
+        character(len=*), intent(in) :: filek
+        integer :: nw
+        real(kind=8) :: pi
+        pi = acos(-1.d0)
+        open(unit=20, file=filek, status='unknown', action='write', position='append')
+        do nw = 1, Ltrot
+            write(20, *) 2.d0 * pi * dble(nw - 1) / Beta
+            write(20, *) real(gw(nw)), aimag(gw(nw))
+        enddo
+        close(20)
+    end subroutine write_w
+
+    ! the files of m_write_obs_tau, in its order
+    subroutine Fourier_prtau(Obs)
+        class(ObserTau), intent(in) :: Obs
+        complex(kind=8), allocatable :: gk2(:,:), gk1(:), gw(:)
+        integer :: izero, ipi, iqx, iqy
+        logical :: has_pi, has_qx, has_qy
+        call Fourier_init()
+        allocate(gk2(Lq, Ltrot), gk1(Lq), gw(Ltrot))
+        izero = 1
+        has_pi = mod(Nlx, 2) == 0 .and. mod(Nly, 2) == 0
+        ipi = -1
+        if (has_pi) ipi = inv_n_list(Nlx / 2 + 1, Nly / 2 + 1)
+        has_qx = Nlx > 1
+        has_qy = Nly > 1
+        iqx = izero
+        iqy = izero
+        if (has_qx) iqx = inv_n_list(2, 1)
+        if (has_qy) iqy = inv_n_list(1, 2)
+
+        call r_to_k(Obs%spin_tau, gk2)
+        call integrate_susc(gk2, gk1)
+        call write_reciprocal(gk1, 'sdw_susc')
+        if (has_pi) call write_k(gk1, 'sdw_pipi_susc', ipi)
+        call write_k(gk1, 'sdw_zero_susc', izero)
+
+        call r_to_k(Obs%charge_d_tau, gk2)
+        if (has_pi) call write_k_tau(gk2, 'cdw_d_pipi_tau', ipi)
+        call integrate_susc(gk2, gk1)
+        call write_reciprocal(gk1, 'cdw_d_susc')
+        if (has_pi) call write_k(gk1, 'cdw_d_pipi_susc', ipi)
+        call write_k(gk1, 'cdw_d_zero_susc', izero)
+
+        call r_to_k(Obs%charge_s_tau, gk2)
+        if (has_pi) call write_k_tau(gk2, 'cdw_s_pipi_tau', ipi)
+        call integrate_susc(gk2, gk1)
+        call write_reciprocal(gk1, 'cdw_s_susc')
+        if (has_pi) call write_k(gk1, 'cdw_s_pipi_susc', ipi)
+        call write_k(gk1, 'cdw_s_zero_susc', izero)
+
+        call write_r_tau(Obs%single_tau, 'g_r0_tau', Lq)
+        call r_to_k(Obs%single_tau, gk2)
+        call integrate_susc(gk2, gk1)
+        call write_reciprocal(gk1, 'g_susc')
+        if (has_pi) call write_k(gk1, 'g_pipi_susc', ipi)
+        call write_k(gk1, 'g_zero_susc', izero)
+
+        call r_to_k(Obs%curxx_tau, gk2)
+        if (has_qx) call write_k_tau(gk2, 'curxx_L_tau', iqx)
+        if (has_qy) call write_k_tau(gk2, 'curxx_T_tau', iqy)
+        call write_k_tau(gk2, 'curxx_zero_tau', izero)
+        call integrate_susc(gk2, gk1)
+        call integrate_susc_freq(gk2, gw)
+        call write_w(gw, 'curxx_freq')
+        call write_reciprocal(gk1, 'curxx_susc')
+        if (has_qx) call write_k(gk1, 'curxx_L_susc', iqx)
+        if (has_qy) call write_k(gk1, 'curxx_T_susc', iqy)
+        call write_k(gk1, 'curxx_zero_susc', izero)
+
+        call r_to_k(Obs%pair_s_tau, gk2)
+        call write_k_tau(gk2, 'pair_s_zero_tau', izero)
+        call integrate_susc(gk2, gk1)
+        call write_reciprocal(gk1, 'pair_s_susc')
+        call write_k(gk1, 'pair_s_zero_susc', izero)
+
+        call r_to_k(Obs%pair_d_tau, gk2)
+        call write_k_tau(gk2, 'pair_d_zero_tau', izero)
+        call integrate_susc(gk2, gk1)
+        call write_reciprocal(gk1, 'pair_d_susc')
+        call write_k(gk1, 'pair_d_zero_susc', izero)
+    end subroutine Fourier_prtau
+end module FourierTrans_mod
+
 module LocalSweep_mod
     use iso_c_binding
     use z2dqmc_c
     use DQMC_Model_mod
     use ProcessMatrix
     use ObserEqual_mod
+    use ObserTau_mod
     implicit none
     type :: LocalSweep
     contains
@@ -248,15 +570,22 @@ module LocalSweep_mod
         procedure, nopass :: ctrl_print_t => Therm_control_print
     end type LocalSweep
     type(ObserEqual), allocatable :: Obs_equal
+    type(ObserTau), allocatable :: Obs_tau
 contains
     subroutine Local_sweep_init(this)
         class(LocalSweep), intent(inout) :: this
         allocate(Obs_equal)
         call Obs_equal%make()
+        if (is_tau) then
+            allocate(Obs_tau)
+            call Obs_tau%make()
+            call z2_check(z2_enable_tau(ctx), "z2_enable_tau")
+        endif
     end subroutine Local_sweep_init
     subroutine Local_sweep_clear(this)
         class(LocalSweep), intent(inout) :: this
         deallocate(Obs_equal)
+        if (allocated(Obs_tau)) deallocate(Obs_tau)
     end subroutine Local_sweep_clear
     ! src/local_sweep.f90:87-104
     subroutine Local_sweep_pre(this, PropU, PropD, WrU, WrD)
@@ -285,7 +614,13 @@ contains
         logical, intent(in) :: toggle
         integer(c_int32_t) :: b, s
         real(c_double) :: xmax, xmean
-        call z2_check(z2_sweep(ctx, Z2_BETA_PER_CHAIN), "z2_sweep")
+        if (toggle) then
+            ! with the time-displaced measurement (Dyn%sweep_R at every right pass)
+            call z2_check(z2_sweep_tau(ctx, Z2_BETA_PER_CHAIN, 1_c_int32_t), "z2_sweep_tau")
+            call Obs_tau%ave(Nsweep)
+        else
+            call z2_check(z2_sweep(ctx, Z2_BETA_PER_CHAIN), "z2_sweep")
+        endif
         call z2_check(z2_get_is_beta(ctx, 0_c_int32_t, b), "z2_get_is_beta")
         is_beta = b /= 0
         call Obs_equal%ave(2 * Ltrot * Nsweep)

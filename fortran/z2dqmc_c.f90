@@ -136,6 +136,31 @@ module z2dqmc_c
             real(c_double), intent(out) :: G(*)
             integer(c_int) :: rc
         end function
+        function z2_enable_tau(ctx) bind(C, name="z2_enable_tau") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_sweep_tau(ctx, is_beta, toggle) bind(C, name="z2_sweep_tau") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: is_beta, toggle
+            integer(c_int) :: rc
+        end function
+        function z2_get_obs_tau(ctx, chain, out) bind(C, name="z2_get_obs_tau") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: out(*)
+            integer(c_int) :: rc
+        end function
+        function z2_set_lambda(ctx, chain, lam) bind(C, name="z2_set_lambda") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(in) :: lam(*)
+            integer(c_int) :: rc
+        end function
         function z2_get_lambda(ctx, chain, lam) bind(C, name="z2_get_lambda") result(rc)
             import :: c_int, c_int32_t, c_ptr, c_double
             type(c_ptr), value :: ctx
