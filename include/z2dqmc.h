@@ -156,6 +156,19 @@ int z2_get_counters(z2_ctx *ctx, int32_t chain, uint64_t out[2]);
  * accepts (default min(16, kmax(N)), kmax set by the LDS size); the decisions
  * do not depend on k, G only through rounding */
 int z2_set_delay(z2_ctx *ctx, int32_t k);
+/* G recompute of the stabilization (no counterpart in the reference):
+ * Z2_STAB_REFERENCE (default) is stab_green (src/stabilization.f90:122-200):
+ * M = (UL UR)^T + DR (VR VL) DL, column-pivoted QR, G = UL^T P V^-1 D^-1 Q^T UR^T,
+ * with the reference's branch on nt > Ltrot/2.  Z2_STAB_DSPLIT is the D-split form
+ * (Loh et al.): DR = DRb DRs, DL = DLb DLs with Db = max(D, 1), Ds = min(D, 1),
+ * M' = DRb^-1 (UL UR)^T DLb^-1 + DRs (VR VL) DLs, G = UL^T DLb^-1 M'^-1 DRb^-1 UR^T:
+ * no product of large and small scales is ever formed, which keeps G accurate at
+ * large beta where the reference's form loses it; it is a deviation from the
+ * reference's arithmetic (G agrees to rounding where both are accurate), for
+ * opt-in use.  The time-displaced recompute is not affected. */
+#define Z2_STAB_REFERENCE 0
+#define Z2_STAB_DSPLIT 1
+int z2_set_stab_mode(z2_ctx *ctx, int32_t mode);
 /* QR used by the stabilization (no counterpart in the reference, which calls
  * ZGEQP3): Z2_QR_BLOCKED (default) orders the columns once by decreasing norm
  * and runs a blocked Householder QR with block-reflector updates on the matrix

@@ -5,5 +5,5 @@ include/z2dqmc.h); this package is the Python host mirror of the reference's
 sweep driver API.
 """
 from ._lib import (Z2_BETA_PER_CHAIN, Z2_PREC_FP64, Z2_PREC_MIXED, Z2_QR_BLOCKED, Z2_QR_PIVOTED,  # noqa: F401
-                   Z2Error, lib)
+                   Z2_STAB_DSPLIT, Z2_STAB_REFERENCE, Z2Error, lib)
 from .engine import ChainSpec, Engine, OBS_NAMES  # noqa: F401

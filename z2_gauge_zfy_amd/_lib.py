@@ -17,6 +17,8 @@ Z2_PREC_MIXED = 1
 Z2_QR_BLOCKED = 0
 Z2_QR_PIVOTED = 1
 Z2_BETA_PER_CHAIN = -1
+Z2_STAB_REFERENCE = 0
+Z2_STAB_DSPLIT = 1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
@@ -78,6 +80,7 @@ _SIGS = {
     "z2_get_lambda": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_set_lambda": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_set_is_beta": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_set_stab_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_get_is_beta": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),
     "z2_global_sweep": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_get_global_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64),

@@ -667,9 +667,10 @@ __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, i
 struct ElemArgs {
   const double *in;
   double *out;
-  const double *d;  // chain stride N
+  const double *d;  // chain stride dstride * N
   const int *p;     // chain stride N
   int N, mode;
+  int dstride;
 };
 
 __global__ void __launch_bounds__(256) k_elem(ElemArgs e) {
@@ -677,7 +678,7 @@ __global__ void __launch_bounds__(256) k_elem(ElemArgs e) {
   const int chain = blockIdx.y;
   const double *in = e.in + (size_t)chain * N * N;
   double *out = e.out + (size_t)chain * N * N;
-  const double *d = e.d ? e.d + (size_t)chain * N : nullptr;
+  const double *d = e.d ? e.d + (size_t)chain * e.dstride * N : nullptr;
   const int *p = e.p ? e.p + (size_t)chain * N : nullptr;
   // a block walks whole columns (no index division; coalesced in r)
   for (int c = blockIdx.x; c < N; c += gridDim.x)
@@ -714,6 +715,37 @@ __global__ void __launch_bounds__(256) k_build_m(const double *X, const double *
     if (trans) m[c + (size_t)r * N] = val;
     else m[idx] = val;
   }
+}
+
+// D-split scales (Loh et al.): for D > 0, Db = max(D, 1), Ds = min(D, 1), so D = Db Ds.
+// out per chain: [0,N) DRb, [N,2N) DRs, [2N,3N) DLb, [3N,4N) DLs.
+__global__ void __launch_bounds__(256) k_dsplit(const double *DR, const double *DL, double *out, int N) {
+  const int chain = blockIdx.x;
+  const double *dr = DR + (size_t)chain * N, *dl = DL + (size_t)chain * N;
+  double *o = out + (size_t)chain * 4 * N;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    o[i] = dr[i] > 1.0 ? dr[i] : 1.0;
+    o[N + i] = dr[i] > 1.0 ? 1.0 : dr[i];
+    o[2 * N + i] = dl[i] > 1.0 ? dl[i] : 1.0;
+    o[3 * N + i] = dl[i] > 1.0 ? 1.0 : dl[i];
+  }
+}
+
+// M' = DRb^{-1} X DLb^{-1} + DRs VRVL DLs (the D-split form of the matrix of
+// stab_green: all entries bounded by the scales' ratio 1, no huge D products)
+__global__ void __launch_bounds__(256) k_build_m_split(const double *X, const double *VRVL, const double *sc,
+                                                       double *M, int N) {
+  const int chain = blockIdx.y;
+  const double *x = X + (size_t)chain * N * N, *v = VRVL + (size_t)chain * N * N;
+  const double *s = sc + (size_t)chain * 4 * N;
+  double *m = M + (size_t)chain * N * N;
+  for (int c = blockIdx.x; c < N; c += gridDim.x)
+    for (int r = threadIdx.x; r < N; r += blockDim.x) {
+      const size_t idx = r + (size_t)c * N;
+      const double a = (x[idx] / s[r]) / s[2 * N + c];
+      const double b = s[N + r] * (v[idx] * s[3 * N + c]);
+      m[idx] = a + b;
+    }
 }
 
 // drift = max |Gnew - Gold| per chain; Xmaxm/Xmeanm update; Gold <- Gnew.

@@ -110,6 +110,8 @@ struct z2_ctx {
   int *ipvt = nullptr;
   double *qrw = nullptr, *qrw2 = nullptr;  // workspaces of the blocked QR (N x N, 2N x 2N)
   int qr_mode = Z2_QR_BLOCKED;
+  int stab_mode = Z2_STAB_REFERENCE;
+  double *dsc = nullptr;  // D-split scales [chain][4][N]
   // mixed precision: propagated G, UUR, UUL and the delayed updates in FP32
   bool mixed = false;
   float *Gf = nullptr, *UURf = nullptr, *UULf = nullptr, *Tbf = nullptr, *Vbf = nullptr;
@@ -236,6 +238,7 @@ static void register_fields(z2_ctx *c) {
   for (double **m : {&c->M, &c->Q, &c->X, &c->Y, &c->Gn, &c->qrw}) reg_field(c, *m, nn, 1, 0);
   reg_field(c, c->tau, N, 1, 0);
   reg_field(c, c->Dtmp, N, 1, 0);
+  reg_field(c, c->dsc, 4 * N, 1, 0);
   reg_field(c, c->ipvt, N, 1, 0);
   reg_field(c, c->rec, (size_t)c->dwords, 1, 0);
   reg_field(c, c->cmp, 1, 1, 0);
@@ -504,6 +507,7 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if ((rc = dalloc(c, &c->Dtmp, B * N))) break;
     if ((rc = dalloc(c, &c->ipvt, B * N))) break;
     if ((rc = dalloc(c, &c->qrw, B * nn))) break;
+    if ((rc = dalloc(c, &c->dsc, B * 4 * (size_t)N))) break;
     c->dwords = (N2 + 31) / 32;
     {
       int dev = 0, lds = 0;
@@ -596,6 +600,12 @@ extern "C" int z2_set_delay(z2_ctx *c, int32_t k) {
   if (!c || k < 1 || k > c->kdelay_max)
     return set_err(Z2_EINVAL, "delay depth must be in [1, %d] for N = %d", c ? c->kdelay_max : 0, c ? c->N : 0);
   c->kdelay = k;
+  return Z2_OK;
+}
+
+extern "C" int z2_set_stab_mode(z2_ctx *c, int32_t mode) {
+  if (!c || (mode != Z2_STAB_REFERENCE && mode != Z2_STAB_DSPLIT)) return set_err(Z2_EINVAL, "bad stabilization mode");
+  c->stab_mode = mode;
   return Z2_OK;
 }
 
@@ -971,7 +981,12 @@ __global__ void __launch_bounds__(256) k_maxdiff(const Ra *A, const Rb *B, doubl
 }
 
 static int elem(z2_ctx *c, const double *in, double *out, int mode, const double *d = nullptr,
-                const int *p = nullptr) {
+                const int *p = nullptr, int dstride = 1);
+// elem with a scale vector of chain stride dstride * N
+static int elem_d(z2_ctx *c, const double *in, double *out, int mode, const double *d, int dstride) {
+  return elem(c, in, out, mode, d, nullptr, dstride);
+}
+static int elem(z2_ctx *c, const double *in, double *out, int mode, const double *d, const int *p, int dstride) {
   ElemArgs e;
   e.in = in;
   e.out = out;
@@ -979,6 +994,7 @@ static int elem(z2_ctx *c, const double *in, double *out, int mode, const double
   e.p = p;
   e.N = c->N;
   e.mode = mode;
+  e.dstride = dstride;
   int nb = (c->N * c->N + 255) / 256;
   if (nb > 64) nb = 64;
   hipLaunchKernelGGL(k_elem, dim3(nb, c->nch), dim3(256), 0, c->stream, e);
@@ -1234,7 +1250,36 @@ static int stab_UL(z2_ctx *c) {
 }
 
 // stab_green, src/stabilization.f90:122-200 -> c->Gn
+// D-split recompute (no counterpart in the reference; opt-in, Z2_STAB_DSPLIT):
+// G = UL^T DLb^{-1} M'^{-1} DRb^{-1} UR^T with M' = DRb^{-1} (UL UR)^T DLb^{-1} + DRs VR VL DLs
+// and M' P = Q D V as in the reference path, so M'^{-1} = P V^{-1} D^{-1} Q^T.
+static int stab_green_split(z2_ctx *c) {
+  CHK(gemm(c, c->VUR, 0, c->VUL, 0, c->Y));
+  CHK(gemm(c, c->UUR, 1, c->UUL, 1, c->X));
+  hipLaunchKernelGGL(k_dsplit, dim3(c->nch), dim3(256), 0, c->stream, (const double *)c->DUR,
+                     (const double *)c->DUL, c->dsc, c->N);
+  LAUNCH_CHECK();
+  {
+    const int nb = c->N < 64 ? c->N : 64;
+    hipLaunchKernelGGL(k_build_m_split, dim3(nb, c->nch), dim3(256), 0, c->stream, (const double *)c->X,
+                       (const double *)c->Y, (const double *)c->dsc, c->M, c->N);
+    LAUNCH_CHECK();
+  }
+  CHK(qrp(c, c->M, c->Dtmp));
+  CHK(orgqr(c, c->M, c->Q));
+  // DRb^{-1} UR^T = (UR DRb^{-1})^T: divide the columns of UR by DRb
+  CHK(elem_d(c, c->UUR, c->Gn, 9, c->dsc, 4));
+  CHK(gemm(c, c->Q, 1, c->Gn, 1, c->X));           // Q^T DRb^{-1} UR^T
+  CHK(elem(c, c->X, c->X, 8, c->Dtmp));            // D^{-1} ...
+  CHK(trsm(c, c->M, c->X, 0));                     // V^{-1} ...
+  CHK(elem(c, c->X, c->Y, 6, nullptr, c->ipvt));   // P ...
+  CHK(elem_d(c, c->Y, c->Y, 8, c->dsc + 2 * (size_t)c->N, 4));  // DLb^{-1} ...
+  CHK(gemm(c, c->UUL, 1, c->Y, 0, c->Gn));         // UL^T ...
+  return Z2_OK;
+}
+
 static int stab_green(z2_ctx *c, int nt) {
+  if (c->stab_mode == Z2_STAB_DSPLIT) return stab_green_split(c);
   CHK(gemm(c, c->VUR, 0, c->VUL, 0, c->Y));
   CHK(gemm(c, c->UUR, 1, c->UUL, 1, c->X));
   const int trans = nt > c->T / 2;

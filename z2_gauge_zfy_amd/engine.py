@@ -196,6 +196,10 @@ class Engine:
     def set_delay(self, k: int):
         check(lib().z2_set_delay(self._ptr, int(k)))
 
+    def set_stab_mode(self, mode: int):
+        """Z2_STAB_REFERENCE (default) or Z2_STAB_DSPLIT (D-split G recompute, opt-in)"""
+        check(lib().z2_set_stab_mode(self._ptr, int(mode)))
+
     def set_qr_mode(self, mode: int):
         """Z2_QR_BLOCKED (columns pre-ordered by norm, blocked) or Z2_QR_PIVOTED (ZGEQP3 pivoting)."""
         check(lib().z2_set_qr_mode(self._ptr, int(mode)))
