@@ -464,18 +464,47 @@ struct GemmArgs {
 #define GT 64   // tile M, N
 #define GK 16   // tile K
 
-// element (r, c) of op(X), op(X) having R rows and Cc columns
-__device__ __forceinline__ double gemm_ld(const double *X, int ld, int trans, int tri, int r, int c, int R,
-                                          int Cc) {
-  if (r >= R || c >= Cc) return 0.0;
-  int sr = trans ? c : r, sc = trans ? r : c;
-  if (tri && sr > sc) return 0.0;
-  return X[sr + (size_t)sc * ld];
-}
-
 // C = alpha op(A) op(B) + beta C, batched over chains (grid.z), FP64 MFMA 16x16x4.
 // trans flags: 0 = N, 1 = T.  triA / triB: use only the upper triangle (row <= col,
-// in stored coordinates) of A / B (ZTRMM as a masked GEMM).
+// in stored coordinates) of A / B (ZTRMM as a masked GEMM).  A 64 x 64 tile of C
+// per workgroup (4 wavefronts of 2 x 2 MFMA blocks); each K step of 16 stages
+// op(A) and op(B) in LDS with lanes running along each operand's contiguous
+// dimension (coalesced for every transpose case), and the next step's values are
+// read into registers while the current one is multiplied.
+struct GemmTileLd {
+  // 4 values per thread per operand and K step: element (i, k) of a GT x GK panel
+  int i[4], k[4];
+};
+
+// staging coordinates of thread t for a panel whose storage is contiguous along
+// the tile dimension (i) or along K
+__device__ __forceinline__ GemmTileLd gemm_coords(int t, bool contig_i) {
+  GemmTileLd c;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int e = t + 256 * q;  // 0 .. GT*GK-1
+    if (contig_i) {
+      c.i[q] = e % GT;
+      c.k[q] = e / GT;
+    } else {
+      c.k[q] = e % GK;
+      c.i[q] = e / GK;
+    }
+  }
+  return c;
+}
+
+// element (r, c) of op(X) (R x Cc), 0 outside or below the diagonal when tri;
+// the address is clamped in range and the value selected (no branch around the load)
+__device__ __forceinline__ double gemm_ld(const double *X, int ld, int trans, int tri, int r, int c, int R,
+                                          int Cc) {
+  const bool in = r < R && c < Cc;
+  const int sr = trans ? c : r, sc = trans ? r : c;
+  const bool keep = in && !(tri && sr > sc);
+  const double v = X[in ? sr + (size_t)sc * ld : 0];
+  return keep ? v : 0.0;
+}
+
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   __shared__ double As[GK][GT + 1];  // As[k][m]
   __shared__ double Bs[GK][GT + 1];  // Bs[k][n]
@@ -486,17 +515,35 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   const int m0 = blockIdx.x * GT, n0 = blockIdx.y * GT;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int wm = (wid & 1) * 32, wn = (wid >> 1) * 32;  // each wave: 32x32 = 2x2 MFMA tiles
+  // op(A)(m, k) is contiguous along m when A is not transposed; op(B)(k, n) is
+  // contiguous along n when B is transposed
+  const GemmTileLd ca = gemm_coords(threadIdx.x, g.ta == 0);
+  const GemmTileLd cb = gemm_coords(threadIdx.x, g.tb != 0);
   typedef double d4 __attribute__((ext_vector_type(4)));
   d4 acc[2][2];
   for (int x = 0; x < 2; ++x)
     for (int y = 0; y < 2; ++y) acc[x][y] = (d4){0.0, 0.0, 0.0, 0.0};
+  double ra[4], rb[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    ra[q] = gemm_ld(A, g.lda, g.ta, g.triA, m0 + ca.i[q], ca.k[q], g.M, g.K);
+    rb[q] = gemm_ld(B, g.ldb, g.tb, g.triB, cb.k[q], n0 + cb.i[q], g.K, g.Nc);
+  }
   for (int k0 = 0; k0 < g.K; k0 += GK) {
-    for (int idx = threadIdx.x; idx < GT * GK; idx += blockDim.x) {
-      int m = idx % GT, k = idx / GT;
-      As[k][m] = gemm_ld(A, g.lda, g.ta, g.triA, m0 + m, k0 + k, g.M, g.K);
-      Bs[k][m] = gemm_ld(B, g.ldb, g.tb, g.triB, k0 + k, n0 + m, g.K, g.Nc);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      As[ca.k[q]][ca.i[q]] = ra[q];
+      Bs[cb.k[q]][cb.i[q]] = rb[q];
     }
     __syncthreads();
+    if (k0 + GK < g.K) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        ra[q] = gemm_ld(A, g.lda, g.ta, g.triA, m0 + ca.i[q], k0 + GK + ca.k[q], g.M, g.K);
+        rb[q] = gemm_ld(B, g.ldb, g.tb, g.triB, k0 + GK + cb.k[q], n0 + cb.i[q], g.K, g.Nc);
+      }
+    }
+#pragma unroll
     for (int kk = 0; kk < GK; kk += 4) {
       const int kl = kk + (lane >> 4), il = lane & 15;
       double av[2], bv[2];
