@@ -607,3 +607,66 @@ __global__ void __launch_bounds__(512) k_orgqr_blk(QrbArgs a) {
   }
   // the first panel's update covered all of Q
 }
+
+// Explicit Q spread over many workgroups per matrix: k_orgqr_t builds the T factor
+// of every panel (one workgroup per panel and matrix) into Tws, then one
+// k_orgqr_apply launch per panel, last panel first, applies that panel's block
+// reflector with one workgroup per column tile.  Each tile sees the same
+// operations as in k_orgqr_blk.  Tws: per matrix npan x NB x NB.
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_orgqr_t(QrbArgs a, double *Tws) {
+  const int N = a.N, tc = a.tc, p = blockIdx.x, chain = blockIdx.y;
+  const int npan = (N + NB - 1) / NB;
+  const double *A = a.Ain + (size_t)chain * N * N;
+  const double *tau = a.tin + (size_t)chain * N;
+  extern __shared__ unsigned char smem_raw[];
+  QrbLds L = qrb_carve(smem_raw, N, NB, tc);
+  const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
+  qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);
+  for (int c = threadIdx.x; c < w; c += blockDim.x) L.s_tau[c] = tau[j0 + c];
+  __syncthreads();
+  qrb_build_t<NB>(L, m, w);
+  double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) T[e] = L.Tm[(e % NB) + (size_t)(e / NB) * L.lw];
+}
+
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_orgqr_apply(QrbArgs a, const double *Tws, int p) {
+  const int N = a.N, tc = a.tc, chain = blockIdx.y;
+  const int npan = (N + NB - 1) / NB;
+  const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
+  const int c0 = j0 + blockIdx.x * tc;
+  if (c0 >= N) return;  // whole workgroup
+  const int tcn = min(tc, N - c0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const double *A = a.Ain + (size_t)chain * N * N;
+  double *Q = a.Q + (size_t)chain * N * N;
+  extern __shared__ unsigned char smem_raw[];
+  QrbLds L = qrb_carve(smem_raw, N, NB, tc);
+  qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);
+  const double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) L.Tm[(e % NB) + (size_t)(e / NB) * L.lw] = T[e];
+  // Q[j0:, c0:c0+tcn]: the later panels wrote Q[j0+w:, j0+w:]; everything else
+  // still is the unit matrix
+  if (c0 >= j0 + w) {
+    qrb_tile_load<RPL>(L.Ct, L.ld, Q + j0 + (size_t)c0 * N, N, nullptr, m, tcn);
+    for (int c = wave; c < tcn; c += nw)
+      for (int r = lane; r < w && r < m; r += 64) L.Ct[(size_t)c * L.ld + r] = 0.0;
+  } else {
+    for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        if (r < m) L.Ct[(size_t)c * L.ld + r] = (j0 + r == c0 + c) ? 1.0 : 0.0;
+      }
+  }
+  __syncthreads();
+  qrb_apply_tile<NB>(L, m, w, tcn, 0);
+  for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = lane + 64 * t;
+      if (r < m) Q[(j0 + r) + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
+    }
+}
+

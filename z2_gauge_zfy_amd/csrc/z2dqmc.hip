@@ -109,6 +109,8 @@ struct z2_ctx {
   double *tau = nullptr, *Dtmp = nullptr;
   int *ipvt = nullptr;
   double *qrw = nullptr, *qrw2 = nullptr;  // workspaces of the blocked QR (N x N, 2N x 2N)
+  double *ortw = nullptr;  // panel T factors of the explicit Q (ceil(N/16) x 32 x 32 per chain)
+  size_t ortw_per = 0;
   int qr_mode = Z2_QR_BLOCKED;
   int stab_mode = Z2_STAB_REFERENCE;
   double *dsc = nullptr;  // D-split scales [chain][4][N]
@@ -239,6 +241,7 @@ static void register_fields(z2_ctx *c) {
   reg_field(c, c->tau, N, 1, 0);
   reg_field(c, c->Dtmp, N, 1, 0);
   reg_field(c, c->dsc, 4 * N, 1, 0);
+  reg_field(c, c->ortw, c->ortw_per, 1, 0);
   reg_field(c, c->ipvt, N, 1, 0);
   reg_field(c, c->rec, (size_t)c->dwords, 1, 0);
   reg_field(c, c->cmp, 1, 1, 0);
@@ -508,6 +511,8 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if ((rc = dalloc(c, &c->ipvt, B * N))) break;
     if ((rc = dalloc(c, &c->qrw, B * nn))) break;
     if ((rc = dalloc(c, &c->dsc, B * 4 * (size_t)N))) break;
+    c->ortw_per = (size_t)((N + 15) / 16) * 32 * 32;
+    if ((rc = dalloc(c, &c->ortw, B * c->ortw_per))) break;
     c->dwords = (N2 + 31) / 32;
     {
       int dev = 0, lds = 0;
@@ -1118,6 +1123,40 @@ static int launch_qrb(int orgqr, int nb, int nmat, size_t sh, hipStream_t st, co
   return nb == 32 ? launch_qrb_nb<32, 0>(rpl, nmat, sh, st, q) : launch_qrb_nb<16, 0>(rpl, nmat, sh, st, q);
 }
 
+// explicit Q over many workgroups per matrix (panel T factors, then one launch per panel)
+template <int NB, int RPL>
+static int launch_orgqr_split_t(int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q, double *Tws) {
+  const int npan = (n + NB - 1) / NB;
+  if (sh > 65536) {
+    HIPC(hipFuncSetAttribute((const void *)k_orgqr_t<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    HIPC(hipFuncSetAttribute((const void *)k_orgqr_apply<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  }
+  hipLaunchKernelGGL((k_orgqr_t<NB, RPL>), dim3(npan, nmat), dim3(512), sh, st, q, Tws);
+  LAUNCH_CHECK();
+  for (int p = npan - 1; p >= 0; --p) {
+    const int ntile = (n - p * NB + tc - 1) / tc;
+    hipLaunchKernelGGL((k_orgqr_apply<NB, RPL>), dim3(ntile, nmat), dim3(512), sh, st, q, (const double *)Tws, p);
+    LAUNCH_CHECK();
+  }
+  return Z2_OK;
+}
+
+template <int NB>
+static int launch_orgqr_split_nb(int rpl, int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q,
+                                 double *Tws) {
+  switch (rpl) {
+    case 1: return launch_orgqr_split_t<NB, 1>(n, tc, nmat, sh, st, q, Tws);
+    case 2: return launch_orgqr_split_t<NB, 2>(n, tc, nmat, sh, st, q, Tws);
+    case 3: return launch_orgqr_split_t<NB, 3>(n, tc, nmat, sh, st, q, Tws);
+    case 4: return launch_orgqr_split_t<NB, 4>(n, tc, nmat, sh, st, q, Tws);
+    case 5: return launch_orgqr_split_t<NB, 5>(n, tc, nmat, sh, st, q, Tws);
+    case 6: return launch_orgqr_split_t<NB, 6>(n, tc, nmat, sh, st, q, Tws);
+    case 7: return launch_orgqr_split_t<NB, 7>(n, tc, nmat, sh, st, q, Tws);
+    case 8: return launch_orgqr_split_t<NB, 8>(n, tc, nmat, sh, st, q, Tws);
+  }
+  return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
+}
+
 static double *qr_workspace(z2_ctx *c, int n) { return n == c->N ? c->qrw : (n == 2 * c->N ? c->qrw2 : nullptr); }
 
 static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n, int nopiv) {
@@ -1166,7 +1205,14 @@ static int orgqr_n(z2_ctx *c, const double *A, const double *tau, double *Qo, in
     q.N = n;
     q.tc = tc;
     CHK(prof_begin(c, Z2_K_ORGQR));
-    CHK(launch_qrb(1, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
+    if (n == c->N && c->ortw) {
+      const int rpl = (n + 63) / 64;
+      const size_t sh = qrb_lds_bytes(n, nb, tc);
+      if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw));
+      else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw));
+    } else {
+      CHK(launch_qrb(1, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
+    }
     CHK(prof_end(c, Z2_K_ORGQR));
     return Z2_OK;
   }
