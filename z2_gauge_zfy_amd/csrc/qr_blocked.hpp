@@ -670,3 +670,126 @@ __global__ void __launch_bounds__(512) k_orgqr_apply(QrbArgs a, const double *Tw
     }
 }
 
+// The QR spread over many workgroups per matrix.  k_qr_order: column norms, the
+// order (decreasing norm, ties by index) into ipvt; the host then copies the
+// ordered columns into W.  Per panel, k_qr_panel (one workgroup per matrix)
+// factors the panel from W, writes its columns of A (normalised R rows of the
+// diagonal block, signs, reflectors), tau, D and the block reflector's T into
+// Tws; k_qr_trail applies it to the trailing columns with one workgroup per
+// column tile: the panel's rows become final normalised R rows in A, the rest go
+// back to W.  Each tile sees the same operations as in k_qr_blk.
+template <int RPL>
+__global__ void __launch_bounds__(512) k_qr_order(QrbArgs a) {
+  const int N = a.N, chain = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const double *A = a.A + (size_t)chain * N * N;
+  int *ipvt = a.ipvt + (size_t)chain * N;
+  extern __shared__ unsigned char smem_raw[];
+  double *nrm = reinterpret_cast<double *>(smem_raw);
+  for (int j0 = wave; j0 < N; j0 += 4 * nw) {
+    double sq[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int j = j0 + nw * u;
+      const double *col = A + (size_t)(j < N ? j : 0) * N;
+      double v[RPL];
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        v[t] = (j < N && r < N) ? col[r] : 0.0;
+      }
+      sq[u] = 0.0;
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) sq[u] += v[t] * v[t];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sq[u] = qrb_wave_sum(sq[u]);
+    if (lane == 0)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (j0 + nw * u < N) nrm[j0 + nw * u] = sq[u];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    int rank = j;
+    if (!a.nopiv) {
+      const double x = nrm[j];
+      rank = 0;
+#pragma unroll 8
+      for (int i = 0; i < N; ++i) {
+        const double y = nrm[i];
+        rank += (y > x) || (y == x && i < j);
+      }
+    }
+    ipvt[rank] = j + 1;
+  }
+}
+
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_qr_panel(QrbArgs a, double *Tws, int p) {
+  const int N = a.N, tc = a.tc, chain = blockIdx.x;
+  const int npan = (N + NB - 1) / NB;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double *A = a.A + (size_t)chain * N * N;
+  const double *W = a.W + (size_t)chain * N * N;
+  double *tau = a.tau + (size_t)chain * N;
+  double *D = a.D + (size_t)chain * N;
+  extern __shared__ unsigned char smem_raw[];
+  QrbLds L = qrb_carve(smem_raw, N, NB, tc);
+  const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
+  qrb_panel_reg<NB, RPL>(L, W + j0 + (size_t)j0 * N, N, nullptr, m, w, N - 1 - j0);
+  if (j0 + w < N) {
+    qrb_build_t<NB>(L, m, w);
+    double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) T[e] = L.Tm[(e % NB) + (size_t)(e / NB) * L.lw];
+  }
+  for (int c = wave; c < w; c += nw)
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = lane + 64 * t;
+      if (r < m) {
+        double x;
+        if (r < c) x = L.Pn[(size_t)c * L.ld + r] / fabs(L.s_beta[r]);
+        else if (r == c) x = L.s_beta[c] / fabs(L.s_beta[c]);
+        else x = L.Pn[(size_t)c * L.ld + r];
+        A[(j0 + r) + (size_t)(j0 + c) * N] = x;
+      }
+    }
+  for (int c = threadIdx.x; c < w; c += blockDim.x) {
+    tau[j0 + c] = L.s_tau[c];
+    D[j0 + c] = fabs(L.s_beta[c]);
+  }
+}
+
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_qr_trail(QrbArgs a, const double *Tws, int p) {
+  const int N = a.N, tc = a.tc, chain = blockIdx.y;
+  const int npan = (N + NB - 1) / NB;
+  const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
+  const int c0 = j0 + w + blockIdx.x * tc;
+  if (c0 >= N) return;  // whole workgroup
+  const int tcn = min(tc, N - c0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double *A = a.A + (size_t)chain * N * N;
+  double *W = a.W + (size_t)chain * N * N;
+  const double *D = a.D + (size_t)chain * N;
+  extern __shared__ unsigned char smem_raw[];
+  QrbLds L = qrb_carve(smem_raw, N, NB, tc);
+  qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);  // reflectors below the diagonal
+  const double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) L.Tm[(e % NB) + (size_t)(e / NB) * L.lw] = T[e];
+  qrb_tile_load<RPL>(L.Ct, L.ld, W + j0 + (size_t)c0 * N, N, nullptr, m, tcn);
+  __syncthreads();
+  qrb_apply_tile<NB>(L, m, w, tcn, 1);
+  for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = lane + 64 * t;
+      if (r < m) {
+        const double x = L.Ct[(size_t)c * L.ld + r];
+        if (r < w) A[(j0 + r) + (size_t)(c0 + c) * N] = x / D[j0 + r];
+        else W[(j0 + r) + (size_t)(c0 + c) * N] = x;
+      }
+    }
+}
+

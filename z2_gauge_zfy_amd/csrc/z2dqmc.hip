@@ -1157,6 +1157,47 @@ static int launch_orgqr_split_nb(int rpl, int n, int tc, int nmat, size_t sh, hi
   return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
 }
 
+// the QR over many workgroups per matrix (order, copy of the ordered columns to
+// W, then per panel: factorization, trailing update)
+template <int NB, int RPL>
+static int launch_qr_split_t(z2_ctx *c, size_t sh, const QrbArgs &q) {
+  const int n = q.N, tc = q.tc, npan = (n + NB - 1) / NB;
+  if (sh > 65536) {
+    HIPC(hipFuncSetAttribute((const void *)k_qr_panel<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    HIPC(hipFuncSetAttribute((const void *)k_qr_trail<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  }
+  hipLaunchKernelGGL((k_qr_order<RPL>), dim3(c->nch), dim3(512), sizeof(double) * n, c->stream, q);
+  LAUNCH_CHECK();
+  CHK(elem(c, q.A, q.W, 5, nullptr, q.ipvt));  // W = A P
+  for (int p = 0; p < npan; ++p) {
+    const int j0 = p * NB, w = n - j0 < NB ? n - j0 : NB;
+    hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, p);
+    LAUNCH_CHECK();
+    if (j0 + w < n) {
+      const int ntile = (n - j0 - w + tc - 1) / tc;
+      hipLaunchKernelGGL((k_qr_trail<NB, RPL>), dim3(ntile, c->nch), dim3(512), sh, c->stream, q,
+                         (const double *)c->ortw, p);
+      LAUNCH_CHECK();
+    }
+  }
+  return Z2_OK;
+}
+
+template <int NB>
+static int launch_qr_split_nb(z2_ctx *c, int rpl, size_t sh, const QrbArgs &q) {
+  switch (rpl) {
+    case 1: return launch_qr_split_t<NB, 1>(c, sh, q);
+    case 2: return launch_qr_split_t<NB, 2>(c, sh, q);
+    case 3: return launch_qr_split_t<NB, 3>(c, sh, q);
+    case 4: return launch_qr_split_t<NB, 4>(c, sh, q);
+    case 5: return launch_qr_split_t<NB, 5>(c, sh, q);
+    case 6: return launch_qr_split_t<NB, 6>(c, sh, q);
+    case 7: return launch_qr_split_t<NB, 7>(c, sh, q);
+    case 8: return launch_qr_split_t<NB, 8>(c, sh, q);
+  }
+  return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
+}
+
 static double *qr_workspace(z2_ctx *c, int n) { return n == c->N ? c->qrw : (n == 2 * c->N ? c->qrw2 : nullptr); }
 
 static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n, int nopiv) {
@@ -1174,7 +1215,14 @@ static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n,
     q.tc = tc;
     q.nopiv = nopiv;
     CHK(prof_begin(c, Z2_K_QRP));
-    CHK(launch_qrb(0, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
+    if (n == c->N && c->ortw) {
+      const int rpl = (n + 63) / 64;
+      const size_t sh = qrb_lds_bytes(n, nb, tc);
+      if (nb == 32) CHK(launch_qr_split_nb<32>(c, rpl, sh, q));
+      else CHK(launch_qr_split_nb<16>(c, rpl, sh, q));
+    } else {
+      CHK(launch_qrb(0, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
+    }
     CHK(prof_end(c, Z2_K_QRP));
     return Z2_OK;
   }
