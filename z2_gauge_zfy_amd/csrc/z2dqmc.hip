@@ -863,9 +863,9 @@ static KinMat<R> kmat(R *m, int nflag, int flush = 0, int pre1 = 0, int pre2 = 0
   return k;
 }
 
-static const int KCB = 16;  // columns per block, row-rotation kernel
+static const int KCB = 8;  // columns per block, row-rotation kernel (8: best measured of 4, 8, 16, 32)
 static constexpr int FLIP_THREADS = 512;  // must not exceed k_flip's __launch_bounds__
-static const int KRB = 16;  // rows per block, column-rotation kernel
+static const int KRB = 8;  // rows per block, column-rotation kernel (8: best measured of 4, 8, 16, 32)
 
 template <typename R>
 static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
