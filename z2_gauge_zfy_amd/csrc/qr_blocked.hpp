@@ -119,17 +119,39 @@ __device__ __forceinline__ double qrb_wave_sum(double v) {
   return qrb_readlane(v, 63);
 }
 
+// C independent wavefront sums by the tree of qrb_wave_sum, stage by stage, so
+// that their latencies overlap (each sum is bit-identical to qrb_wave_sum's)
+template <int C>
+__device__ __forceinline__ void qrb_wave_sum_n(double (&v)[C]) {
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] += qrb_dpp<0xB1>(v[q]);
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] += qrb_dpp<0x4E>(v[q]);
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] += qrb_dpp<0x141>(v[q]);
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] += qrb_dpp<0x140>(v[q]);
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] += qrb_dpp_rows<0x142, 0xA>(v[q]);
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] += qrb_dpp_rows<0x143, 0xC>(v[q]);
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] = qrb_readlane(v[q], 63);
+}
+
 // Householder factorization of a panel (m x w, w <= NB) in registers: reflectors
 // below the diagonal, alpha kept on it (beta in s_beta), tau in s_tau.  Wavefront w owns columns
 // c = w + 8 q, lane l rows r = l + 64 t (RPL = ceil(N / 64)).  One barrier per
-// column; the reflector reaches the other wavefronts through LDS (double-buffered),
-// and the owner of column k+1 computes its reflector right after applying
-// reflector k to it, while the other wavefronts are still updating (look-ahead).
+// column; the reflector reaches the other wavefronts through LDS (double-buffered).
+// Each wavefront applies reflector k to all its trailing columns with one
+// interleaved set of wavefront sums, and the owner of column k+1 then computes
+// reflector k+1 before the barrier.
 // Reads the panel (m x w) from global memory: column c at Wp + col(c) * ldW with
 // col(c) = cperm[c] (or c); leaves the factored panel in L.Pn.
 template <int NB, int RPL>
 __device__ __forceinline__ void qrb_reflector(QrbLds &L, double (&x)[NB / 8][RPL], int k, int m, int last_row,
                                               int lane) {
+#pragma clang fp contract(fast)
   constexpr int NW = 8, CPW = NB / NW;
   const int qk = k / NW;
   double col[RPL];
@@ -183,6 +205,7 @@ __device__ __forceinline__ void qrb_reflector(QrbLds &L, double (&x)[NB / 8][RPL
 template <int NB, int RPL>
 __device__ void qrb_panel_reg(QrbLds &L, const double *Wp, int ldW, const int *cperm, int m, int w,
                               int last_row) {
+#pragma clang fp contract(fast)
   constexpr int NW = 8, CPW = NB / NW;  // 512 threads
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double x[CPW][RPL];
@@ -207,54 +230,29 @@ __device__ void qrb_panel_reg(QrbLds &L, const double *Wp, int ldW, const int *c
       const int r = lane + 64 * t;
       vr[t] = qrb_ld(vb, (size_t)r, r < m);
     }
-    const int k1 = k + 1;
-    const bool own1 = k1 < w && wave == k1 % NW;
-    if (own1) {
-      // column k+1 first, then its reflector (published for the next step)
-      const int q1 = k1 / NW;
-      if (tk != 0.0) {
-        double col[RPL], d = 0.0;
-#pragma unroll
-        for (int t = 0; t < RPL; ++t) {
-          col[t] = x[0][t];
-#pragma unroll
-          for (int q = 1; q < CPW; ++q)
-            if (q == q1) col[t] = x[q][t];
-          d += vr[t] * col[t];
-        }
-        d = qrb_wave_sum(d);
-        const double tt = -tk * d;
-#pragma unroll
-        for (int q = 0; q < CPW; ++q)
-          if (q == q1)
-#pragma unroll
-            for (int t = 0; t < RPL; ++t) x[q][t] = col[t] + vr[t] * tt;
-      }
-      qrb_reflector<NB, RPL>(L, x, k1, m, last_row, lane);
-    }
     if (tk != 0.0) {
       double d[CPW];
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
         d[q] = 0.0;
         const int c = wave + NW * q;
-        if (c > k1 || (c == k1 && !own1))
-          if (c < w)
+        if (c > k && c < w)
 #pragma unroll
-            for (int t = 0; t < RPL; ++t) d[q] += vr[t] * x[q][t];
+          for (int t = 0; t < RPL; ++t) d[q] += vr[t] * x[q][t];
       }
-#pragma unroll
-      for (int q = 0; q < CPW; ++q) d[q] = qrb_wave_sum(d[q]);
+      qrb_wave_sum_n<CPW>(d);
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
         const int c = wave + NW * q;
-        if ((c > k1 || (c == k1 && !own1)) && c < w) {
+        if (c > k && c < w) {
           const double tt = -tk * d[q];
 #pragma unroll
           for (int t = 0; t < RPL; ++t) x[q][t] = x[q][t] + vr[t] * tt;
         }
       }
     }
+    const int k1 = k + 1;
+    if (k1 < w && wave == k1 % NW) qrb_reflector<NB, RPL>(L, x, k1, m, last_row, lane);
     __syncthreads();
   }
 #pragma unroll
@@ -331,38 +329,56 @@ __device__ void qrb_vt_times(QrbLds &L, int m, int w, int ncol, int bv, double *
   }
 }
 
-// T (w x w, upper triangular) of the panel's block reflector by the ZLARFT
-// recurrence T[0:i, i] = -tau_i T[0:i, 0:i] (V^T v_i)[0:i], T[i][i] = tau_i.
-// Wavefront 0 holds row j of T and of the Gram matrix in lane j's registers.
+// One doubling level of the block reflector factor: every pair of H x H diagonal
+// blocks (T1 at rows/columns o..o+H-1, T2 at o+H..o+2H-1) is merged into
+// [T1, -T1 (V1^T V2) T2; 0, T2].  V1^T V2 is the Gram block above the diagonal
+// of W1 (entries of reflectors >= w read as zero); Y = (V1^T V2) T2 goes to the
+// transposed positions below the diagonal of W1, which no later level reads.
+template <int NB, int H>
+__device__ __forceinline__ void qrb_t_level(QrbLds &L, int w) {
+  constexpr int B = 2 * H, CNT = (NB / B) * H * H;
+  const int lw = L.lw;
+  for (int e = threadIdx.x; e < CNT; e += blockDim.x) {
+    const int blk = e / (H * H), rr = (e / H) % H, cc = e % H;  // constant divisors
+    const int o = blk * B, r = o + rr, c = o + H + cc;
+    double y = 0.0;
+#pragma unroll
+    for (int l = 0; l < H; ++l) {  // T2 is upper triangular: zero for l > cc
+      const int g = o + H + l;
+      const double G = (r < w && g < w) ? L.W1[r + (size_t)g * lw] : 0.0;
+      y += G * L.Tm[g + (size_t)c * lw];
+    }
+    L.W1[c + (size_t)r * lw] = y;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < CNT; e += blockDim.x) {
+    const int blk = e / (H * H), rr = (e / H) % H, cc = e % H;
+    const int o = blk * B, r = o + rr, c = o + H + cc;
+    double x = 0.0;
+#pragma unroll
+    for (int l = 0; l < H; ++l)  // T1 is upper triangular: zero for l < rr
+      x += L.Tm[r + (size_t)(o + l) * lw] * L.W1[c + (size_t)(o + l) * lw];
+    L.Tm[r + (size_t)c * lw] = -x;
+  }
+  __syncthreads();
+}
+
+// T (NB x NB, upper triangular, zero at indices >= w) of the panel's block
+// reflector H_0 ... H_{w-1} = I - V T V^T: diagonal tau_i, then log2(NB)
+// doubling levels over the Gram matrix V^T V (the recursive form of ZLARFT).
 template <int NB>
 __device__ void qrb_build_t(QrbLds &L, int m, int w) {
   qrb_vt_times(L, m, w, w, 1, L.W1);  // Gram
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (wave == 0) {
-    const int j = lane;
-    double U[NB], T[NB];
-#pragma unroll
-    for (int l = 0; l < NB; ++l) {
-      U[l] = (j < w && l < w && l > j) ? L.W1[j + (size_t)l * L.lw] : 0.0;
-      T[l] = 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      if (i < w) {
-        const double ti = L.s_tau[i];
-        double s = 0.0;
-#pragma unroll
-        for (int l = 0; l < i; ++l) s += T[l] * qrb_readlane(U[i], l);
-        T[i] = (j < i) ? -ti * s : (j == i ? ti : 0.0);
-      }
-    }
-    if (j < NB) {
-#pragma unroll
-      for (int l = 0; l < NB; ++l) L.Tm[j + (size_t)l * L.lw] = (j < w && l < w) ? T[l] : 0.0;
-    }
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
+    const int i = e % NB, l = e / NB;
+    L.Tm[i + (size_t)l * L.lw] = (i == l && i < w) ? L.s_tau[i] : 0.0;
   }
   __syncthreads();
+  qrb_t_level<NB, 1>(L, w);
+  qrb_t_level<NB, 2>(L, w);
+  qrb_t_level<NB, 4>(L, w);
+  qrb_t_level<NB, 8>(L, w);
+  if constexpr (NB >= 32) qrb_t_level<NB, 16>(L, w);
 }
 
 // C (m x tcn tile in L.Ct) <- (I - V op(T) V^T) C, op(T) = T^T (trans = 1) or T
