@@ -796,15 +796,19 @@ __global__ void __launch_bounds__(256) k_build_m_split(const double *X, const do
 }
 
 // drift = max |Gnew - Gold| per chain; Xmaxm/Xmeanm update; Gold <- Gnew.
-// One block per chain.
+// gridDim.x workgroups per chain, each over a contiguous part of the matrix; the
+// partial maxima go to part[chain * pstride + b] and k_drift_finish takes their
+// maximum (exact, so the split does not change the result).
 template <typename R>
-__global__ void __launch_bounds__(256) k_drift_replace(const double *Gnew, R *Gold, double *xmax,
-                                                       double *xmean, int N, int accumulate) {
-  const int chain = blockIdx.x;
-  const double *gn = Gnew + (size_t)chain * N * N;
-  R *go = Gold + (size_t)chain * N * N;
+__global__ void __launch_bounds__(256) k_drift_replace(const double *Gnew, R *Gold, double *part, int pstride,
+                                                       int N) {
+  const int chain = blockIdx.y;
+  const size_t nn = (size_t)N * N, per = (nn + gridDim.x - 1) / gridDim.x;
+  const size_t lo = (size_t)blockIdx.x * per, hi = lo + per < nn ? lo + per : nn;
+  const double *gn = Gnew + (size_t)chain * nn;
+  R *go = Gold + (size_t)chain * nn;
   double m = 0.0;
-  for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+  for (size_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
     double d = fabs(gn[idx] - (double)go[idx]);
     m = d > m ? d : m;
     go[idx] = (R)gn[idx];
@@ -820,7 +824,19 @@ __global__ void __launch_bounds__(256) k_drift_replace(const double *Gnew, R *Go
   if (threadIdx.x == 0) {
     double mm = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) mm = red[w] > mm ? red[w] : mm;
-    if (mm > xmax[chain]) xmax[chain] = mm;
-    if (accumulate) xmean[chain] += mm;
+    part[(size_t)chain * pstride + blockIdx.x] = mm;
   }
+}
+
+__global__ void __launch_bounds__(64) k_drift_finish(const double *part, int pstride, int nb, double *xmax,
+                                                     double *xmean, int nch, int accumulate) {
+  const int chain = blockIdx.x * blockDim.x + threadIdx.x;
+  if (chain >= nch) return;
+  double mm = 0.0;
+  for (int b = 0; b < nb; ++b) {
+    const double v = part[(size_t)chain * pstride + b];
+    mm = v > mm ? v : mm;
+  }
+  if (mm > xmax[chain]) xmax[chain] = mm;
+  if (accumulate) xmean[chain] += mm;
 }

@@ -30,7 +30,7 @@ struct ObsArgs {
 // The N x N pair sums (pair_sc_q0, cdw_pi, cdw_shift) of one slice, one 32 x 32
 // tile of (i, j) pairs per workgroup: G(i, j) and G(j, i) of the tile are staged
 // in LDS by coalesced column reads; the tile's three partial sums go to `part`
-// and k_obs_finish adds them in tile order (deterministic).
+// and k_obs_finish adds them by a fixed tree (deterministic).
 #define OBS_TB 32
 template <typename R>
 __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
   }
 }
 
-// density and flux of the slice, the tile partials in tile order, and the
+// density and flux of the slice, the sum of the tile partials, and the
 // accumulation into the bin (one workgroup of 256 threads per chain)
 template <typename R>
 __global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
@@ -105,16 +105,19 @@ __global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
     const int *pl = a.lat.plaq + 4 * i;
     flux += (double)((int)s[pl[0]] * (int)s[pl[1]] * (int)s[pl[2]] * (int)s[pl[3]]);
   }
+  const double *p = a.part + (size_t)chain * a.part_stride * 3;
+  double pair = 0.0, cdw = 0.0, cdws = 0.0;
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+    pair += p[3 * t];
+    cdw += p[3 * t + 1];
+    cdws += p[3 * t + 2];
+  }
   dens = block_reduce_sum(dens, red);
   flux = block_reduce_sum(flux, red);
+  pair = block_reduce_sum(pair, red);
+  cdw = block_reduce_sum(cdw, red);
+  cdws = block_reduce_sum(cdws, red);
   if (threadIdx.x == 0) {
-    const double *p = a.part + (size_t)chain * a.part_stride * 3;
-    double pair = 0.0, cdw = 0.0, cdws = 0.0;
-    for (int t = 0; t < ntile; ++t) {
-      pair += p[3 * t];
-      cdw += p[3 * t + 1];
-      cdws += p[3 * t + 2];
-    }
     double *acc = a.acc + (size_t)chain * OBS_NACC;
     const double ft = flux / (double)N;
     acc[OBS_FLUX_SUM] += ft;

@@ -1403,12 +1403,17 @@ static int stab_green(z2_ctx *c, int nt) {
 }
 
 static int drift_replace(z2_ctx *c, int accumulate) {
+  // partial maxima in Dtmp (N per chain), free once the recomputed G is in Gn
+  const int nb = c->N < 16 ? c->N : 16;
   if (c->mixed)
-    hipLaunchKernelGGL((k_drift_replace<float>), dim3(c->nch), dim3(256), 0, c->stream, (const double *)c->Gn,
-                       c->Gf, c->xmax, c->xmean, c->N, accumulate);
+    hipLaunchKernelGGL((k_drift_replace<float>), dim3(nb, c->nch), dim3(256), 0, c->stream, (const double *)c->Gn,
+                       c->Gf, c->Dtmp, c->N, c->N);
   else
-    hipLaunchKernelGGL((k_drift_replace<double>), dim3(c->nch), dim3(256), 0, c->stream, (const double *)c->Gn,
-                       c->G, c->xmax, c->xmean, c->N, accumulate);
+    hipLaunchKernelGGL((k_drift_replace<double>), dim3(nb, c->nch), dim3(256), 0, c->stream, (const double *)c->Gn,
+                       c->G, c->Dtmp, c->N, c->N);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_drift_finish, dim3((c->nch + 63) / 64), dim3(64), 0, c->stream, (const double *)c->Dtmp,
+                     c->N, nb, c->xmax, c->xmean, c->nch, accumulate);
   LAUNCH_CHECK();
   return Z2_OK;
 }
