@@ -32,6 +32,8 @@ struct FlipArgs {
   int mu_on_read;         // 1: G entries are read as (g*ep)*em (left pass, propT_L before flips)
   int kdelay;             // delay depth: pending updates before an in-kernel flush
   int *mu_done;           // out: 1 when an in-kernel flush applied the mu factors to G
+  int final_flush;        // FP32 only, 1: flush the updates still pending at the end into G and
+                          // write (g*ep)*em (right pass, LocalK_prop_R then propT_R); nacc is then 0
   const uint32_t *force;  // optional: decisions to replay, bit k = proposal k, chain stride dwords
   uint32_t *record;       // optional: decisions taken, same layout
   int dwords;
@@ -186,6 +188,85 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   // (B) build the rank-2 update of the first accepted one.  The values a
   // proposal sees are bit-identical to those of the serial loop.
   int start = 0, na = 0, last = -1, cur = 0, round = 0, tot = 0;
+  // FP32 flush of the na pending updates into G (mu_w: write (g*ep)*em), used in
+  // the round loop and, for the right pass, once more at the end; the FP64 flush
+  // stays inline in the round loop
+  auto flush32 = [&](const bool mu_w) {
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, nw = nth >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    constexpr int BPW = FLIP_BPW;
+    const int nbk = (N + 15) >> 4, ncg = (nbk + BPW - 1) / BPW, K = 2 * na;
+    // FP32: O = -T V^T, item q of lane (li, lk) is G(16 rb + 4 lk + q, col):
+    // four consecutive rows of one column, one 16-byte access per tile
+    // the tiles of the wavefront's next task are requested before this task's
+    // products and stores, so waiting for them never waits for the stores
+    const int ntask = nbk * ncg;
+    auto load_tiles = [&](int task, float4 *tile) {
+      const int row0 = 16 * (task % nbk) + 4 * lk;  // N is a multiple of 4: a float4 is all in or all out
+      const int cb0 = BPW * 16 * (task / nbk);
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+        const int col = cb0 + 16 * u + li;
+        const bool in = row0 < N && col < N;
+        tile[u] = *reinterpret_cast<const float4 *>(G + (in ? row0 + (size_t)col * N : 0));  // selected at use
+      }
+    };
+    float4 cur[BPW], nxt[BPW];
+    if (wave < ntask) load_tiles(wave, cur);
+    for (int task = wave; task < ntask; task += nw) {
+      if (task + nw < ntask) load_tiles(task + nw, nxt);
+      const int rb = task % nbk;
+      const int cb0 = BPW * 16 * (task / nbk);
+      const int row0 = 16 * rb + 4 * lk;
+      typename MfmaAcc<R>::type acc[BPW];
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+        const bool in = row0 < N && cb0 + 16 * u + li < N;
+        float x0 = in ? cur[u].x : 0.f, x1 = in ? cur[u].y : 0.f, x2 = in ? cur[u].z : 0.f, x3 = in ? cur[u].w : 0.f;
+        if (mu) {
+          x0 = x0 * (float)ep;
+          x0 = x0 * (float)em;
+          x1 = x1 * (float)ep;
+          x1 = x1 * (float)em;
+          x2 = x2 * (float)ep;
+          x2 = x2 * (float)em;
+          x3 = x3 * (float)ep;
+          x3 = x3 * (float)em;
+        }
+        acc[u][0] = x0;
+        acc[u][1] = x1;
+        acc[u][2] = x2;
+        acc[u][3] = x3;
+      }
+      const int r = 16 * rb + li;
+      for (int kk = 0; kk < K; kk += 4) {
+        const int kq = kk + lk;
+        const double *P = PT + (size_t)4 * N * (kq >> 1) + (size_t)(kq & 1) * N;
+        const R av = (kq < K && r < N) ? (R)P[r] : (R)0;  // T_kq(r)
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          const int c = cb0 + 16 * u + li;
+          const R bv = (kq < K && c < N) ? (R)(-P[2 * N + c]) : (R)0;  // -V_kq(c)
+          acc[u] = mfma_16x16x4<R>(av, bv, acc[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) {
+        const int col = cb0 + 16 * u + li;
+        if (mu_w)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            acc[u][q] = acc[u][q] * (float)ep;
+            acc[u][q] = acc[u][q] * (float)em;
+          }
+        if (row0 < N && col < N)
+          *reinterpret_cast<float4 *>(G + row0 + (size_t)col * N) =
+              make_float4(acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
+      }
+#pragma unroll
+      for (int u = 0; u < BPW; ++u) cur[u] = nxt[u];
+    }
+  };
   while (true) {
     const int slot = round & 1;
     if (na >= a.kdelay) {
@@ -199,70 +280,7 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       constexpr int BPW = FLIP_BPW;
       const int nbk = (N + 15) >> 4, ncg = (nbk + BPW - 1) / BPW, K = 2 * na;
       if constexpr (sizeof(R) == 4) {
-        // FP32: O = -T V^T, item q of lane (li, lk) is G(16 rb + 4 lk + q, col):
-        // four consecutive rows of one column, one 16-byte access per tile
-        // the tiles of the wavefront's next task are requested before this task's
-        // products and stores, so waiting for them never waits for the stores
-        const int ntask = nbk * ncg;
-        auto load_tiles = [&](int task, float4 *tile) {
-          const int row0 = 16 * (task % nbk) + 4 * lk;  // N is a multiple of 4: a float4 is all in or all out
-          const int cb0 = BPW * 16 * (task / nbk);
-#pragma unroll
-          for (int u = 0; u < BPW; ++u) {
-            const int col = cb0 + 16 * u + li;
-            const bool in = row0 < N && col < N;
-            tile[u] = *reinterpret_cast<const float4 *>(G + (in ? row0 + (size_t)col * N : 0));  // selected at use
-          }
-        };
-        float4 cur[BPW], nxt[BPW];
-        if (wave < ntask) load_tiles(wave, cur);
-        for (int task = wave; task < ntask; task += nw) {
-          if (task + nw < ntask) load_tiles(task + nw, nxt);
-          const int rb = task % nbk;
-          const int cb0 = BPW * 16 * (task / nbk);
-          const int row0 = 16 * rb + 4 * lk;
-          typename MfmaAcc<R>::type acc[BPW];
-#pragma unroll
-          for (int u = 0; u < BPW; ++u) {
-            const bool in = row0 < N && cb0 + 16 * u + li < N;
-            float x0 = in ? cur[u].x : 0.f, x1 = in ? cur[u].y : 0.f, x2 = in ? cur[u].z : 0.f, x3 = in ? cur[u].w : 0.f;
-            if (mu) {
-              x0 = x0 * (float)ep;
-              x0 = x0 * (float)em;
-              x1 = x1 * (float)ep;
-              x1 = x1 * (float)em;
-              x2 = x2 * (float)ep;
-              x2 = x2 * (float)em;
-              x3 = x3 * (float)ep;
-              x3 = x3 * (float)em;
-            }
-            acc[u][0] = x0;
-            acc[u][1] = x1;
-            acc[u][2] = x2;
-            acc[u][3] = x3;
-          }
-          const int r = 16 * rb + li;
-          for (int kk = 0; kk < K; kk += 4) {
-            const int kq = kk + lk;
-            const double *P = PT + (size_t)4 * N * (kq >> 1) + (size_t)(kq & 1) * N;
-            const R av = (kq < K && r < N) ? (R)P[r] : (R)0;  // T_kq(r)
-#pragma unroll
-            for (int u = 0; u < BPW; ++u) {
-              const int c = cb0 + 16 * u + li;
-              const R bv = (kq < K && c < N) ? (R)(-P[2 * N + c]) : (R)0;  // -V_kq(c)
-              acc[u] = mfma_16x16x4<R>(av, bv, acc[u]);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < BPW; ++u) {
-            const int col = cb0 + 16 * u + li;
-            if (row0 < N && col < N)
-              *reinterpret_cast<float4 *>(G + row0 + (size_t)col * N) =
-                  make_float4(acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
-          }
-#pragma unroll
-          for (int u = 0; u < BPW; ++u) cur[u] = nxt[u];
-        }
+        flush32(false);
       } else {
         for (int task = wave; task < nbk * ncg; task += nw) {
           const int r = 16 * (task % nbk) + li;
@@ -465,6 +483,12 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
     na = na + 1;
     start = kmin + 1;
     round = round + 1;
+  }
+  if constexpr (sizeof(R) == 4) {
+    if (a.final_flush) {
+      flush32(true);
+      na = 0;
+    }
   }
   // the pending updates left for the next kinetic kernel
   for (int idx = tid; idx < na * N; idx += nth) {

@@ -904,7 +904,7 @@ static int launch_elem_kin(z2_ctx *c, KinArgs<R> &a, int nmat) {
 }
 
 template <typename R>
-static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
+static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read, int final_flush = 0) {
   FlipArgs<R> f;
   f.G = PB<R>::G(c);
   f.sig = c->sig;
@@ -922,6 +922,7 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read) {
   f.mu_on_read = mu_on_read;
   f.kdelay = c->kdelay;
   f.mu_done = c->mu_done;
+  f.final_flush = final_flush;
   f.force = c->force;
   f.record = c->rec;
   f.dwords = c->dwords;
@@ -1504,9 +1505,17 @@ static int slice_R_impl(z2_ctx *c, int nt) {
   CHK(launch_rows<R>(c, a, 1));
   a.m[0] = kmat<R>(PB<R>::G(c), -1);                         // G <- G B^-1
   CHK(launch_cols<R>(c, a, 1));
-  CHK(launch_flip<R>(c, nt, 0, 0));
-  a.m[0] = kmat<R>(PB<R>::G(c), 0, 1, 0, 0, FAC_EP, FAC_EM);  // G <- (flush(G)*ep)*em
-  CHK(launch_elem_kin<R>(c, a, 1));
+  // G <- (flush(G)*ep)*em: in FP32 with in-kernel flushes on, by the flip kernel's own
+  // matrix-core flush; in FP64 (the FP64 matrix rate is the vector rate, so the
+  // elementwise kernel over many workgroups per chain is faster) or with the delay
+  // depth covering the slice (updates applied one by one, in acceptance order), by
+  // k_kin_elem
+  const int in_kernel = sizeof(R) == 4 && c->kdelay < c->N2;
+  CHK(launch_flip<R>(c, nt, 0, 0, in_kernel));
+  if (!in_kernel) {
+    a.m[0] = kmat<R>(PB<R>::G(c), 0, 1, 0, 0, FAC_EP, FAC_EM);
+    CHK(launch_elem_kin<R>(c, a, 1));
+  }
   a.m[0] = kmat<R>(PB<R>::UUR(c), +1, 0, 0, 0, 0, 0, FAC_EP);  // UUR <- (B UUR)*ep
   CHK(launch_rows<R>(c, a, 1));
   return Z2_OK;
