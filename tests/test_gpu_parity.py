@@ -77,6 +77,17 @@ def test_parity_batched_uneven_couplings_and_mu():
     _run_parity(6, 20, 2.0, 5, specs, nsweeps=3)
 
 
+@pytest.mark.parametrize("L", [16, 20])
+def test_parity_larger_lattices(L):
+    """The benchmark's lattice, L=16 (N=256: one proposal per flip thread, QR
+    panels of 32 columns split over many workgroups), and L=20 (N=400: two
+    proposals per flip thread, QR panels of 16 columns), Ltrot=20, beta=2,
+    Nwrap=10, two chains (mu = 0 and 0.2), one full sweep with its recomputes."""
+    import z2_gauge_zfy_amd as z
+    specs = [z.ChainSpec(h=0.5, mu=0.2 * k, Beta=2.0, seed=9001 + 2 * 7919 * k) for k in range(2)]
+    _run_parity(L, 20, 2.0, 10, specs, nsweeps=1)
+
+
 @pytest.mark.parametrize("kdelay", [1, 3, 32, 72])
 def test_slice_kernels(kdelay):
     """wraps + flips + delayed updates of single slices, no stabilization.  The
