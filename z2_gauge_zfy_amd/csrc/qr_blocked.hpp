@@ -11,11 +11,11 @@
 // application (compact WY, I - V T V^T) on the FP64 matrix cores instead of one
 // pass over the trailing matrix per column.
 //
-// Panel: NB columns factorized in LDS (one barrier per column, wavefront
-// reductions through DPP).  T by the ZLARFT recurrence from the Gram matrix
-// V^T V, held in the registers of one wavefront.  Trailing columns: tiles of tc
-// columns staged in LDS, W = V^T C, W = T^T W, C -= V W, the two products with
-// v_mfma_f64_16x16x4f64.  LDS leading dimensions are odd so that the 16 columns
+// Panel: NB columns factorized in registers (one barrier per column, wavefront
+// reductions through DPP), then stored in LDS.  T by recursive doubling over the
+// Gram matrix V^T V (the recursive form of ZLARFT) in LDS.  Trailing columns:
+// tiles of tc columns staged in LDS, W = V^T C, W = T^T W, C -= V W, the two
+// products with v_mfma_f64_16x16x4f64.  LDS leading dimensions are odd so that the 16 columns
 // an MFMA operand spans fall in different banks.
 #pragma once
 #include "z2_common.hpp"
@@ -170,16 +170,25 @@ __device__ __forceinline__ void qrb_reflector(QrbLds &L, double (&x)[NB / 8][RPL
     if (t == (k >> 6)) ak = col[t];
   }
   s = qrb_wave_sum(s);
-  const double xnorm = sqrt(s), alpha = qrb_readlane(ak, k & 63);
+  const double alpha = qrb_readlane(ak, k & 63);
   double tau, beta, scale;
-  if (xnorm == 0.0 || k == last_row) {
+  if (s == 0.0 || k == last_row) {
     tau = 0.0;
     beta = alpha;
     scale = 1.0;
   } else {
-    const double xa = fabs(alpha), w_ = fmax(xa, xnorm), z_ = fmin(xa, xnorm);
-    const double q_ = z_ / w_;
-    const double nrm = w_ * sqrt(1.0 + q_ * q_);
+    // |(alpha, x)|: one square root while alpha^2 + |x|^2 is far from overflow and
+    // underflow (the column step's latency is this scalar chain), else the scaled
+    // form w sqrt(1 + (z/w)^2)
+    const double a2 = alpha * alpha + s;
+    double nrm;
+    if (a2 > 1e-290 && a2 < 1e290) {
+      nrm = sqrt(a2);
+    } else {
+      const double xnorm = sqrt(s), xa = fabs(alpha), w_ = fmax(xa, xnorm), z_ = fmin(xa, xnorm);
+      const double q_ = z_ / w_;
+      nrm = w_ * sqrt(1.0 + q_ * q_);
+    }
     beta = -copysign(nrm, alpha);
     tau = (beta - alpha) / beta;
     scale = 1.0 / (alpha - beta);
