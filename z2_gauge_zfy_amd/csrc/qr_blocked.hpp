@@ -305,21 +305,30 @@ __device__ __forceinline__ void qrb_tile_load(double *dst, int ldd, const double
 }
 
 // out (w x ncol, ld L.lw) = V^T B, B = the tile Ct (bv = 0) or V itself (bv = 1).
-// One wavefront per 16x16 output block, two accumulators over K.
+// One wavefront per 16x16 output block, two accumulators over K; with at most
+// half as many blocks as wavefronts, two wavefronts share a block, each over half
+// of K, and the second half's sum is added through out (one more barrier).
 __device__ void qrb_vt_times(QrbLds &L, int m, int w, int ncol, int bv, double *out) {
   constexpr int NW = 8;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lk = lane >> 4;
-  const int nbi = (w + 15) >> 4, nbj = (ncol + 15) >> 4;
+  const int nbi = (w + 15) >> 4, nbj = (ncol + 15) >> 4, nb = nbi * nbj;
   const int ld = L.ld;
-  for (int blk = wave; blk < nbi * nbj; blk += NW) {
+  const bool split = 2 * nb <= NW;
+  for (int blk = split ? wave % nb : wave; blk < nb && (!split || wave < 2 * nb); blk += NW) {
     const int bi = blk % nbi, bj = blk / nbi;
     const int col = 16 * bj + li;
     const int kbeg = 16 * bi;  // V is zero above its diagonal
+    int klo = kbeg, khi = m;
+    if (split) {
+      const int steps = (m - kbeg + 15) >> 4, hs = (steps + 1) >> 1;
+      if (wave < nb) khi = min(m, kbeg + 16 * hs);
+      else klo = kbeg + 16 * hs;
+    }
     qrb_d4 acc[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) acc[u] = (qrb_d4){0.0, 0.0, 0.0, 0.0};
-    for (int kk = kbeg; kk < m; kk += 16) {
+    for (int kk = klo; kk < khi; kk += 16) {
       double av[4], bw[4];  // four K steps of operands loaded together, four chains
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -330,12 +339,31 @@ __device__ void qrb_vt_times(QrbLds &L, int m, int w, int ncol, int bv, double *
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bw[u], acc[u], 0, 0, 0);
     }
+    if (split) {
+      // (rows beyond m in the second half read as zero through qrb_v / qrb_ld)
+      if (wave >= nb)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int ri = 16 * bi + 4 * q + lk, cj = 16 * bj + li;
-      out[ri + (size_t)cj * L.lw] = (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+        for (int q = 0; q < 4; ++q) {
+          const int ri = 16 * bi + 4 * q + lk, cj = 16 * bj + li;
+          out[ri + (size_t)cj * L.lw] = (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+        }
+      __syncthreads();
+      if (wave < nb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int ri = 16 * bi + 4 * q + lk, cj = 16 * bj + li;
+          const double lo = (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+          out[ri + (size_t)cj * L.lw] = lo + out[ri + (size_t)cj * L.lw];
+        }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int ri = 16 * bi + 4 * q + lk, cj = 16 * bj + li;
+        out[ri + (size_t)cj * L.lw] = (acc[0][q] + acc[1][q]) + (acc[2][q] + acc[3][q]);
+      }
     }
   }
+  if (split && wave >= 2 * nb) __syncthreads();  // the idle wavefronts meet the barrier above
 }
 
 // One doubling level of the block reflector factor: every pair of H x H diagonal
@@ -399,35 +427,31 @@ __device__ void qrb_apply_tile(QrbLds &L, int m, int w, int tcn, int trans) {
   const int ld = L.ld, lw = L.lw;
   qrb_vt_times(L, m, w, tcn, 0, L.W1);
   __syncthreads();
-  // W1 <- op(T) W1: entries e = tid + 512 u (u < 2 covers NB x tc <= 1024), into
-  // registers, written back after everyone has read W1.  T is stored with its
-  // zeros (triangle, rows/columns >= w) and W1 has rows up to lmax, so the sums
-  // run unconditionally.
+  // W1 <- op(T) W1 on the matrix cores: wavefront b < (NB/16) x ceil(tcn/16)
+  // computes output block b over K = lmax (T is stored with its zeros: triangle,
+  // rows/columns >= w; W1 has rows up to lmax), written back after everyone has
+  // read W1.  Result item q of lane l is block element (4q + l/16, l%16).
   const int lmax = 16 * ((w + 15) >> 4);
-  double v[2] = {0.0, 0.0};
-#pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int e = threadIdx.x + 512 * u, i = e % NB, j = min(e / NB, tcn - 1);
-    const double *wj = L.W1 + (size_t)j * lw;
-    double part[4] = {0.0, 0.0, 0.0, 0.0};  // four interleaved partial sums, 8 terms per batch
-    for (int l0 = 0; l0 < lmax; l0 += 8) {     // lmax is a multiple of 16
-      double tv[8], wv[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int l = l0 + q;
-        tv[q] = trans ? L.Tm[l + (size_t)i * lw] : L.Tm[i + (size_t)l * lw];
-        wv[q] = wj[l];
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q) part[q & 3] += tv[q] * wv[q];
+  constexpr int TBI = NB / 16;
+  const int tbj = (tcn + 15) >> 4;
+  const bool tw = wave < TBI * tbj;
+  qrb_d4 tacc = (qrb_d4){0.0, 0.0, 0.0, 0.0};
+  if (tw) {
+    const int bi = wave % TBI, bj = wave / TBI;
+    const int i = 16 * bi + li, j = min(16 * bj + li, tcn - 1);
+    for (int l0 = 0; l0 < lmax; l0 += 4) {
+      const int l = l0 + lk;
+      const double av = trans ? L.Tm[l + (size_t)i * lw] : L.Tm[i + (size_t)l * lw];
+      const double bv = L.W1[l + (size_t)j * lw];
+      tacc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, tacc, 0, 0, 0);
     }
-    v[u] = (part[0] + part[1]) + (part[2] + part[3]);
   }
   __syncthreads();
+  if (tw) {
+    const int bi = wave % TBI, bj = wave / TBI, col = 16 * bj + li;
 #pragma unroll
-  for (int u = 0; u < 2; ++u) {
-    const int e = threadIdx.x + 512 * u, i = e % NB, j = e / NB;
-    if (j < tcn) L.W1[i + (size_t)j * lw] = v[u];
+    for (int q = 0; q < 4; ++q)
+      if (col < tcn) L.W1[(16 * bi + 4 * q + lk) + (size_t)col * lw] = tacc[q];
   }
   __syncthreads();
   // C -= V W1: a wavefront takes row blocks wave and wave + 8 (then + 16, ...) and
