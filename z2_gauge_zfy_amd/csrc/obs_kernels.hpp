@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
   __shared__ double A[OBS_TB][OBS_TB + 1];  // A[v][u] = G(i0 + u, j0 + v)
   __shared__ double B[OBS_TB][OBS_TB + 1];  // B[u][v] = G(j0 + v, i0 + u)
   __shared__ double ni[OBS_TB], nj[OBS_TB];
-  __shared__ double red[16];
+  __shared__ double red[48];
   const int tid = threadIdx.x;
   for (int e = tid; e < OBS_TB * OBS_TB; e += blockDim.x) {
     const int c = e / OBS_TB, r = e % OBS_TB;  // constant divisor
@@ -78,14 +78,13 @@ __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
     cdw += ph * dc;
     cdws += a.cosT[dx + W * dy] * dc;
   }
-  pair = block_reduce_sum(pair, red);
-  cdw = block_reduce_sum(cdw, red);
-  cdws = block_reduce_sum(cdws, red);
+  double sums[3] = {pair, cdw, cdws};
+  block_reduce_sum_n<double, 3>(sums, red);
   if (tid == 0) {
     double *p = a.part + ((size_t)chain * a.part_stride + blockIdx.x) * 3;
-    p[0] = pair;
-    p[1] = cdw;
-    p[2] = cdws;
+    p[0] = sums[0];
+    p[1] = sums[1];
+    p[2] = sums[2];
   }
 }
 
@@ -97,7 +96,7 @@ __global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
   const int chain = blockIdx.x, ntile = ((N + OBS_TB - 1) / OBS_TB) * ((N + OBS_TB - 1) / OBS_TB);
   const R *G = a.G + (size_t)chain * N * N;
   const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
-  __shared__ double red[16];
+  __shared__ double red[80];
   double dens = 0.0, flux = 0.0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     const double nd = 1.0 - (double)G[i + (size_t)i * N];
@@ -112,11 +111,13 @@ __global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
     cdw += p[3 * t + 1];
     cdws += p[3 * t + 2];
   }
-  dens = block_reduce_sum(dens, red);
-  flux = block_reduce_sum(flux, red);
-  pair = block_reduce_sum(pair, red);
-  cdw = block_reduce_sum(cdw, red);
-  cdws = block_reduce_sum(cdws, red);
+  double sums[5] = {dens, flux, pair, cdw, cdws};
+  block_reduce_sum_n<double, 5>(sums, red);
+  dens = sums[0];
+  flux = sums[1];
+  pair = sums[2];
+  cdw = sums[3];
+  cdws = sums[4];
   if (threadIdx.x == 0) {
     double *acc = a.acc + (size_t)chain * OBS_NACC;
     const double ft = flux / (double)N;

@@ -33,6 +33,31 @@ struct LatticeDev {
 
 __device__ __forceinline__ int kin_group_of_stage(int s) { return s < 4 ? s : 7 - s; }
 
+// C sums at once, each by the tree of block_reduce_sum (bit-identical to C calls of
+// it); sh holds 16 C values
+template <typename T, int C>
+__device__ __forceinline__ void block_reduce_sum_n(T (&v)[C], T *sh) {
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int q = 0; q < C; ++q) v[q] += __shfl_down(v[q], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < C; ++q) sh[16 * q + wid] = v[q];
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  if (threadIdx.x < C) {
+    T r = 0;
+    for (int w = 0; w < nw; ++w) r += sh[16 * threadIdx.x + w];
+    sh[16 * threadIdx.x] = r;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < C; ++q) v[q] = sh[16 * q];
+  __syncthreads();
+}
+
 template <typename T>
 __device__ __forceinline__ T block_reduce_sum(T v, T *sh) {
   // 64-wide wavefront reduction then across waves via LDS
