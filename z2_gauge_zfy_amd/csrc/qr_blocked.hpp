@@ -726,7 +726,9 @@ __global__ void __launch_bounds__(512) k_orgqr_apply(QrbArgs a, const double *Tw
 // diagonal block, signs, reflectors), tau, D and the block reflector's T into
 // Tws; k_qr_trail applies it to the trailing columns with one workgroup per
 // column tile: the panel's rows become final normalised R rows in A, the rest go
-// back to W.  Each tile sees the same operations as in k_qr_blk.
+// back to W; with tiles as wide as panels, the workgroup of the first tile then
+// factors the next panel (look-ahead), so a panel step is one launch.  Each tile
+// sees the same operations as in k_qr_blk.
 template <int RPL>
 __global__ void __launch_bounds__(512) k_qr_order(QrbArgs a) {
   const int N = a.N, chain = blockIdx.x;
@@ -774,19 +776,20 @@ __global__ void __launch_bounds__(512) k_qr_order(QrbArgs a) {
   }
 }
 
+// Panel p of one matrix: factored from the m x w block at Wp (column stride ldW,
+// global or LDS) into L.Pn; writes its columns of A (normalised R rows of the
+// diagonal block, signs, reflectors), tau, D and, if columns follow, its T into Tws.
 template <int NB, int RPL>
-__global__ void __launch_bounds__(512) k_qr_panel(QrbArgs a, double *Tws, int p) {
-  const int N = a.N, tc = a.tc, chain = blockIdx.x;
+__device__ void qrb_panel_step(const QrbArgs &a, QrbLds &L, double *Tws, int chain, int p, const double *Wp,
+                               int ldW) {
+  const int N = a.N;
   const int npan = (N + NB - 1) / NB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double *A = a.A + (size_t)chain * N * N;
-  const double *W = a.W + (size_t)chain * N * N;
   double *tau = a.tau + (size_t)chain * N;
   double *D = a.D + (size_t)chain * N;
-  extern __shared__ unsigned char smem_raw[];
-  QrbLds L = qrb_carve(smem_raw, N, NB, tc);
   const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
-  qrb_panel_reg<NB, RPL>(L, W + j0 + (size_t)j0 * N, N, nullptr, m, w, N - 1 - j0);
+  qrb_panel_reg<NB, RPL>(L, Wp, ldW, nullptr, m, w, N - 1 - j0);
   if (j0 + w < N) {
     qrb_build_t<NB>(L, m, w);
     double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
@@ -811,7 +814,19 @@ __global__ void __launch_bounds__(512) k_qr_panel(QrbArgs a, double *Tws, int p)
 }
 
 template <int NB, int RPL>
-__global__ void __launch_bounds__(512) k_qr_trail(QrbArgs a, const double *Tws, int p) {
+__global__ void __launch_bounds__(512) k_qr_panel(QrbArgs a, double *Tws, int p) {
+  const int N = a.N, chain = blockIdx.x;
+  extern __shared__ unsigned char smem_raw[];
+  QrbLds L = qrb_carve(smem_raw, N, NB, a.tc);
+  const int j0 = p * NB;
+  qrb_panel_step<NB, RPL>(a, L, Tws, chain, p, a.W + (size_t)chain * N * N + j0 + (size_t)j0 * N, N);
+}
+
+// lookahead (tile width == NB): the workgroup of tile 0, whose columns are the
+// next panel's, goes on to factor panel p + 1 from its updated tile in LDS while
+// the other tiles are updated (the next launch is then panel p + 1's trail).
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_qr_trail(QrbArgs a, double *Tws, int p, int lookahead) {
   const int N = a.N, tc = a.tc, chain = blockIdx.y;
   const int npan = (N + NB - 1) / NB;
   const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
@@ -840,5 +855,9 @@ __global__ void __launch_bounds__(512) k_qr_trail(QrbArgs a, const double *Tws, 
         else W[(j0 + r) + (size_t)(c0 + c) * N] = x;
       }
     }
+  if (lookahead && blockIdx.x == 0) {
+    __syncthreads();  // V of panel p (Pn), T and W1 are free; the tile stays in Ct
+    qrb_panel_step<NB, RPL>(a, L, Tws, chain, p + 1, L.Ct + w, L.ld);
+  }
 }
 

@@ -1170,14 +1170,18 @@ static int launch_qr_split_t(z2_ctx *c, size_t sh, const QrbArgs &q) {
   hipLaunchKernelGGL((k_qr_order<RPL>), dim3(c->nch), dim3(512), sizeof(double) * n, c->stream, q);
   LAUNCH_CHECK();
   CHK(elem(c, q.A, q.W, 5, nullptr, q.ipvt));  // W = A P
+  const int lookahead = tc == NB;  // tile 0 holds exactly the next panel's columns
+  hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, 0);
+  LAUNCH_CHECK();
   for (int p = 0; p < npan; ++p) {
     const int j0 = p * NB, w = n - j0 < NB ? n - j0 : NB;
-    hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, p);
+    if (j0 + w >= n) break;
+    const int ntile = (n - j0 - w + tc - 1) / tc;
+    hipLaunchKernelGGL((k_qr_trail<NB, RPL>), dim3(ntile, c->nch), dim3(512), sh, c->stream, q, c->ortw, p,
+                       lookahead);
     LAUNCH_CHECK();
-    if (j0 + w < n) {
-      const int ntile = (n - j0 - w + tc - 1) / tc;
-      hipLaunchKernelGGL((k_qr_trail<NB, RPL>), dim3(ntile, c->nch), dim3(512), sh, c->stream, q,
-                         (const double *)c->ortw, p);
+    if (!lookahead) {
+      hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, p + 1);
       LAUNCH_CHECK();
     }
   }
