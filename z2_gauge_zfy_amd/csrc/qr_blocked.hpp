@@ -658,10 +658,8 @@ __global__ void __launch_bounds__(512) k_orgqr_blk(QrbArgs a) {
 }
 
 // Explicit Q spread over many workgroups per matrix: k_orgqr_t builds the T factor
-// of every panel (one workgroup per panel and matrix) into Tws, then one
-// k_orgqr_apply launch per panel, last panel first, applies that panel's block
-// reflector with one workgroup per column tile.  Each tile sees the same
-// operations as in k_orgqr_blk.  Tws: per matrix npan x NB x NB.
+// of every panel (one workgroup per panel and matrix) into Tws, then k_orgqr_tiles
+// applies the block reflectors to each column tile.  Tws: per matrix npan x NB x NB.
 template <int NB, int RPL>
 __global__ void __launch_bounds__(512) k_orgqr_t(QrbArgs a, double *Tws) {
   const int N = a.N, tc = a.tc, p = blockIdx.x, chain = blockIdx.y;
@@ -679,43 +677,45 @@ __global__ void __launch_bounds__(512) k_orgqr_t(QrbArgs a, double *Tws) {
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) T[e] = L.Tm[(e % NB) + (size_t)(e / NB) * L.lw];
 }
 
+// One column tile of Q per workgroup, all panels applied to it in one launch:
+// Q[:, tile] = H_0 (H_1 (... (H_last e_tile))); the tile (all N rows) stays in LDS,
+// panel p's block reflector touches rows j0.. of it, and panels that start after
+// the tile's last column leave it unchanged.  The tiles with the most panels are
+// dispatched first.  Each panel application is the same as in k_orgqr_blk.
 template <int NB, int RPL>
-__global__ void __launch_bounds__(512) k_orgqr_apply(QrbArgs a, const double *Tws, int p) {
+__global__ void __launch_bounds__(512) k_orgqr_tiles(QrbArgs a, const double *Tws) {
   const int N = a.N, tc = a.tc, chain = blockIdx.y;
-  const int npan = (N + NB - 1) / NB;
-  const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
-  const int c0 = j0 + blockIdx.x * tc;
-  if (c0 >= N) return;  // whole workgroup
-  const int tcn = min(tc, N - c0);
+  const int npan = (N + NB - 1) / NB, ntile = (N + tc - 1) / tc;
+  const int c0 = (ntile - 1 - (int)blockIdx.x) * tc, tcn = min(tc, N - c0);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   const double *A = a.Ain + (size_t)chain * N * N;
   double *Q = a.Q + (size_t)chain * N * N;
   extern __shared__ unsigned char smem_raw[];
   QrbLds L = qrb_carve(smem_raw, N, NB, tc);
-  qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);
-  const double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
-  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) L.Tm[(e % NB) + (size_t)(e / NB) * L.lw] = T[e];
-  // Q[j0:, c0:c0+tcn]: the later panels wrote Q[j0+w:, j0+w:]; everything else
-  // still is the unit matrix
-  if (c0 >= j0 + w) {
-    qrb_tile_load<RPL>(L.Ct, L.ld, Q + j0 + (size_t)c0 * N, N, nullptr, m, tcn);
-    for (int c = wave; c < tcn; c += nw)
-      for (int r = lane; r < w && r < m; r += 64) L.Ct[(size_t)c * L.ld + r] = 0.0;
-  } else {
-    for (int c = wave; c < tcn; c += nw)
-#pragma unroll
-      for (int t = 0; t < RPL; ++t) {
-        const int r = lane + 64 * t;
-        if (r < m) L.Ct[(size_t)c * L.ld + r] = (j0 + r == c0 + c) ? 1.0 : 0.0;
-      }
-  }
-  __syncthreads();
-  qrb_apply_tile<NB>(L, m, w, tcn, 0);
   for (int c = wave; c < tcn; c += nw)
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {
       const int r = lane + 64 * t;
-      if (r < m) Q[(j0 + r) + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
+      if (r < N) L.Ct[(size_t)c * L.ld + r] = (r == c0 + c) ? 1.0 : 0.0;
+    }
+  const int plast = min(npan - 1, (c0 + tcn - 1) / NB);
+  for (int p = plast; p >= 0; --p) {
+    const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
+    __syncthreads();  // the previous panel's application is done with Pn and Tm
+    qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);
+    const double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) L.Tm[(e % NB) + (size_t)(e / NB) * L.lw] = T[e];
+    __syncthreads();
+    QrbLds Lp = L;
+    Lp.Ct = L.Ct + j0;  // rows j0.. of the tile
+    qrb_apply_tile<NB>(Lp, m, w, tcn, 0);
+  }
+  __syncthreads();
+  for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+    for (int t = 0; t < RPL; ++t) {
+      const int r = lane + 64 * t;
+      if (r < N) Q[r + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
     }
 }
 

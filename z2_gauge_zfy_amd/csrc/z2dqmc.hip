@@ -1130,15 +1130,13 @@ static int launch_orgqr_split_t(int n, int tc, int nmat, size_t sh, hipStream_t 
   const int npan = (n + NB - 1) / NB;
   if (sh > 65536) {
     HIPC(hipFuncSetAttribute((const void *)k_orgqr_t<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-    HIPC(hipFuncSetAttribute((const void *)k_orgqr_apply<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    HIPC(hipFuncSetAttribute((const void *)k_orgqr_tiles<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   }
   hipLaunchKernelGGL((k_orgqr_t<NB, RPL>), dim3(npan, nmat), dim3(512), sh, st, q, Tws);
   LAUNCH_CHECK();
-  for (int p = npan - 1; p >= 0; --p) {
-    const int ntile = (n - p * NB + tc - 1) / tc;
-    hipLaunchKernelGGL((k_orgqr_apply<NB, RPL>), dim3(ntile, nmat), dim3(512), sh, st, q, (const double *)Tws, p);
-    LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL((k_orgqr_tiles<NB, RPL>), dim3((n + tc - 1) / tc, nmat), dim3(512), sh, st, q,
+                     (const double *)Tws);
+  LAUNCH_CHECK();
   return Z2_OK;
 }
 
