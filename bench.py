@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--extra-kernels", default="qrp,orgqr,kin_rows,kin_cols",
                     help="kernel classes timed in one extra sweep each (roofline_other)")
     ap.add_argument("--delay", type=int, default=0, help="delay depth k (0: engine default)")
-    ap.add_argument("--cpu-slices", type=int, default=40)
+    ap.add_argument("--cpu-slices", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-shadow", action="store_true")
     ap.add_argument("--progress", action="store_true", help="one stderr line per sweep")
