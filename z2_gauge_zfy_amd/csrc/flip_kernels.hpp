@@ -200,6 +200,88 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
     // four consecutive rows of one column, one 16-byte access per tile
     // the tiles of the wavefront's next task are requested before this task's
     // products and stores, so waiting for them never waits for the stores
+    if (MPT < 4 && K <= 32 && ncg <= nw && nw % ncg == 0) {  // (registers: not with 4 proposals per thread)
+      // a wavefront keeps one group of BPW column blocks: its -V operands (K <= 32:
+      // eight K steps) are read from LDS once into registers, then its row blocks
+      // are swept; every output block sees the same products in the same order
+      const int wpg = nw / ncg, g = wave % ncg, rb0 = wave / ncg;
+      const int cb0 = BPW * 16 * g;
+      float bvr[8][BPW];
+#pragma unroll
+      for (int st = 0; st < 8; ++st) {
+        const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
+        const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          const int c = cb0 + 16 * u + li;
+          const float v = (float)(-P[2 * N + (c < N ? c : 0)]);
+          bvr[st][u] = (kq < K && c < N) ? v : 0.f;  // -V_kq(c)
+        }
+      }
+      auto load_rb = [&](int rb, float4 *tile) {
+        const int row0 = 16 * rb + 4 * lk;
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          const int col = cb0 + 16 * u + li;
+          const bool in = row0 < N && col < N;
+          tile[u] = *reinterpret_cast<const float4 *>(G + (in ? row0 + (size_t)col * N : 0));  // selected at use
+        }
+      };
+      float4 cur[BPW], nxt[BPW];
+      if (rb0 < nbk) load_rb(rb0, cur);
+      for (int rb = rb0; rb < nbk; rb += wpg) {
+        if (rb + wpg < nbk) load_rb(rb + wpg, nxt);
+        const int row0 = 16 * rb + 4 * lk;
+        typename MfmaAcc<R>::type acc[BPW];
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          const bool in = row0 < N && cb0 + 16 * u + li < N;
+          float x0 = in ? cur[u].x : 0.f, x1 = in ? cur[u].y : 0.f, x2 = in ? cur[u].z : 0.f, x3 = in ? cur[u].w : 0.f;
+          if (mu) {
+            x0 = x0 * (float)ep;
+            x0 = x0 * (float)em;
+            x1 = x1 * (float)ep;
+            x1 = x1 * (float)em;
+            x2 = x2 * (float)ep;
+            x2 = x2 * (float)em;
+            x3 = x3 * (float)ep;
+            x3 = x3 * (float)em;
+          }
+          acc[u][0] = x0;
+          acc[u][1] = x1;
+          acc[u][2] = x2;
+          acc[u][3] = x3;
+        }
+        const int r = 16 * rb + li;
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+          if (4 * st < K) {
+            const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
+            const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
+            const float t = (float)P[r < N ? r : 0];
+            const R av = (kq < K && r < N) ? t : (R)0;  // T_kq(r)
+#pragma unroll
+            for (int u = 0; u < BPW; ++u) acc[u] = mfma_16x16x4<R>(av, bvr[st][u], acc[u]);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) {
+          const int col = cb0 + 16 * u + li;
+          if (mu_w)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              acc[u][q] = acc[u][q] * (float)ep;
+              acc[u][q] = acc[u][q] * (float)em;
+            }
+          if (row0 < N && col < N)
+            *reinterpret_cast<float4 *>(G + row0 + (size_t)col * N) =
+                make_float4(acc[u][0], acc[u][1], acc[u][2], acc[u][3]);
+        }
+#pragma unroll
+        for (int u = 0; u < BPW; ++u) cur[u] = nxt[u];
+      }
+      return;
+    }
     const int ntask = nbk * ncg;
     auto load_tiles = [&](int task, float4 *tile) {
       const int row0 = 16 * (task % nbk) + 4 * lk;  // N is a multiple of 4: a float4 is all in or all out
