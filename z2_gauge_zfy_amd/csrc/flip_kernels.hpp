@@ -364,6 +364,63 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       if constexpr (sizeof(R) == 4) {
         flush32(false);
       } else {
+        bool done = false;
+        if constexpr (MPT == 1) {
+          if (K <= 32 && ncg <= nw && nw % ncg == 0) {
+            // as in the FP32 flush: a wavefront keeps one column group and its -V
+            // operands in registers for the whole flush (same products, same order)
+            const int wpg = nw / ncg, g = wave % ncg;
+            const int cb0 = BPW * 16 * g;
+            double avr[8][BPW];
+  #pragma unroll
+            for (int st = 0; st < 8; ++st) {
+              const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
+              const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
+  #pragma unroll
+              for (int u = 0; u < BPW; ++u) {
+                const int c = cb0 + 16 * u + li;
+                const double v = -P[2 * N + (c < N ? c : 0)];
+                avr[st][u] = (kq < K && c < N) ? v : 0.0;  // -V_kq(c)
+              }
+            }
+            for (int rb = wave / ncg; rb < nbk; rb += wpg) {
+              const int r = 16 * rb + li;
+              typename MfmaAcc<R>::type acc[BPW];
+  #pragma unroll
+              for (int u = 0; u < BPW; ++u)
+  #pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  const int col = cb0 + 16 * u + mfma_row<R>(q, lk);
+                  R x = (r < N && col < N) ? G[r + (size_t)col * N] : (R)0;
+                  if (mu) {
+                    x = x * (R)ep;
+                    x = x * (R)em;
+                  }
+                  acc[u][q] = x;
+                }
+  #pragma unroll
+              for (int st = 0; st < 8; ++st) {
+                if (4 * st < K) {
+                  const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
+                  const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
+                  const double t = P[r < N ? r : 0];
+                  const R bv = (kq < K && r < N) ? (R)t : (R)0;  // T_kq(r)
+  #pragma unroll
+                  for (int u = 0; u < BPW; ++u) acc[u] = mfma_16x16x4<R>((R)avr[st][u], bv, acc[u]);
+                }
+              }
+  #pragma unroll
+              for (int u = 0; u < BPW; ++u)
+  #pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                  const int col = cb0 + 16 * u + mfma_row<R>(q, lk);
+                  if (r < N && col < N) G[r + (size_t)col * N] = acc[u][q];
+                }
+            }
+            done = true;
+          }
+        }
+        if (!done)
         for (int task = wave; task < nbk * ncg; task += nw) {
           const int r = 16 * (task % nbk) + li;
           const int cb0 = BPW * 16 * (task / nbk);
