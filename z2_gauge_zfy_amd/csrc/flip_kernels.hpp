@@ -100,6 +100,14 @@ __device__ __forceinline__ int wave_min_int(int v) {
 
 // MPT: proposals per thread (2N <= 512 MPT); a thread keeps the lattice data,
 // random number and current 2x2 block of its proposals in registers.
+// position of element i of vector c (0, 1) of a pending update's T (V: after 2N):
+// interleaved (2 i + c) in FP64, where the update vectors read the two together,
+// blocked (c N + i) in FP32, where the flush's operand reads stay unit-stride
+template <typename R>
+__device__ __forceinline__ int pt_ix(int N, int c, int i) {
+  return sizeof(R) == 8 ? 2 * i + c : c * N + i;
+}
+
 template <typename R, int MPT>
 __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   const int N = a.lat.N, N2 = a.lat.N2;
@@ -120,7 +128,8 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   double *o12 = dgb + 2 * N;                           // 2N    : G_p1p2 per bond
   double *o21 = o12 + N2;                              // 2N    : G_p2p1 per bond
   double *rnd = o21 + N2;                              // 2N    : random number per proposal index
-  double *PT = rnd + N2;                               // kdelay x 4N : pending T0,T1,V0,V1
+  double *PT = rnd + N2;                               // kdelay x 4N : pending updates, slot k:
+                                                       // T_c(i) at 4Nk + 2i + c, V_c(i) at 4Nk + 2N + 2i + c
   double *bos = PT + (size_t)4 * N * a.kdelay;         // 9 (16) : boson ratio table
   short *bp = reinterpret_cast<short *>(bos + 16);     // 2N x 2 : bond sites
   short *bbs = bp + 2 * N2;                            // 2N x 6 : plaquette partner bonds
@@ -210,11 +219,12 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
 #pragma unroll
       for (int st = 0; st < 8; ++st) {
         const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
-        const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
+        const double *P = PT + (size_t)4 * N * (kc >> 1);
+        const int pc = kc & 1;
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
           const int c = cb0 + 16 * u + li;
-          const float v = (float)(-P[2 * N + (c < N ? c : 0)]);
+          const float v = (float)(-P[2 * N + pt_ix<R>(N, pc, c < N ? c : 0)]);
           bvr[st][u] = (kq < K && c < N) ? v : 0.f;  // -V_kq(c)
         }
       }
@@ -257,8 +267,9 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
         for (int st = 0; st < 8; ++st) {
           if (4 * st < K) {
             const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
-            const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
-            const float t = (float)P[r < N ? r : 0];
+            const double *P = PT + (size_t)4 * N * (kc >> 1);
+            const int pc = kc & 1;
+            const float t = (float)P[pt_ix<R>(N, pc, r < N ? r : 0)];
             const R av = (kq < K && r < N) ? t : (R)0;  // T_kq(r)
 #pragma unroll
             for (int u = 0; u < BPW; ++u) acc[u] = mfma_16x16x4<R>(av, bvr[st][u], acc[u]);
@@ -323,12 +334,13 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       const int r = 16 * rb + li;
       for (int kk = 0; kk < K; kk += 4) {
         const int kq = kk + lk;
-        const double *P = PT + (size_t)4 * N * (kq >> 1) + (size_t)(kq & 1) * N;
-        const R av = (kq < K && r < N) ? (R)P[r] : (R)0;  // T_kq(r)
+        const double *P = PT + (size_t)4 * N * (kq >> 1);
+        const int pc = kq & 1;
+        const R av = (kq < K && r < N) ? (R)P[pt_ix<R>(N, pc, r)] : (R)0;  // T_kq(r)
 #pragma unroll
         for (int u = 0; u < BPW; ++u) {
           const int c = cb0 + 16 * u + li;
-          const R bv = (kq < K && c < N) ? (R)(-P[2 * N + c]) : (R)0;  // -V_kq(c)
+          const R bv = (kq < K && c < N) ? (R)(-P[2 * N + pt_ix<R>(N, pc, c)]) : (R)0;  // -V_kq(c)
           acc[u] = mfma_16x16x4<R>(av, bv, acc[u]);
         }
       }
@@ -375,11 +387,12 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   #pragma unroll
             for (int st = 0; st < 8; ++st) {
               const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
-              const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
+              const double *P = PT + (size_t)4 * N * (kc >> 1);
+              const int pc = kc & 1;
   #pragma unroll
               for (int u = 0; u < BPW; ++u) {
                 const int c = cb0 + 16 * u + li;
-                const double v = -P[2 * N + (c < N ? c : 0)];
+                const double v = -P[2 * N + pt_ix<R>(N, pc, c < N ? c : 0)];
                 avr[st][u] = (kq < K && c < N) ? v : 0.0;  // -V_kq(c)
               }
             }
@@ -402,8 +415,9 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
               for (int st = 0; st < 8; ++st) {
                 if (4 * st < K) {
                   const int kq = 4 * st + lk, kc = kq < K ? kq : 0;
-                  const double *P = PT + (size_t)4 * N * (kc >> 1) + (size_t)(kc & 1) * N;
-                  const double t = P[r < N ? r : 0];
+                  const double *P = PT + (size_t)4 * N * (kc >> 1);
+                  const int pc = kc & 1;
+                  const double t = P[pt_ix<R>(N, pc, r < N ? r : 0)];
                   const R bv = (kq < K && r < N) ? (R)t : (R)0;  // T_kq(r)
   #pragma unroll
                   for (int u = 0; u < BPW; ++u) acc[u] = mfma_16x16x4<R>((R)avr[st][u], bv, acc[u]);
@@ -439,12 +453,13 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
             }
           for (int kk = 0; kk < K; kk += 4) {
             const int kq = kk + lk;
-            const double *P = PT + (size_t)4 * N * (kq >> 1) + (size_t)(kq & 1) * N;
-            const R bv = (kq < K && r < N) ? (R)P[r] : (R)0;  // T_kq(r)
+            const double *P = PT + (size_t)4 * N * (kq >> 1);
+            const int pc = kq & 1;
+            const R bv = (kq < K && r < N) ? (R)P[pt_ix<R>(N, pc, r)] : (R)0;  // T_kq(r)
   #pragma unroll
             for (int u = 0; u < BPW; ++u) {
               const int c = cb0 + 16 * u + li;
-              const R av = (kq < K && c < N) ? (R)(-P[2 * N + c]) : (R)0;  // -V_kq(c)
+              const R av = (kq < K && c < N) ? (R)(-P[2 * N + pt_ix<R>(N, pc, c)]) : (R)0;  // -V_kq(c)
               acc[u] = mfma_16x16x4<R>(av, bv, acc[u]);
             }
           }
@@ -467,7 +482,7 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
     const double *VA = last >= 0 ? TA + 2 * N : nullptr;
     if (last >= 0)
       for (int i = tid; i < N; i += nth) {
-        double d = TA[i] * VA[i] + TA[N + i] * VA[N + i];
+        double d = TA[pt_ix<R>(N, 0, i)] * VA[pt_ix<R>(N, 0, i)] + TA[pt_ix<R>(N, 1, i)] * VA[pt_ix<R>(N, 1, i)];
         dgn[i] = dgc[i] - d;
       }
     int cand = 2 * N2 + 2;
@@ -477,14 +492,14 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       if (k >= N2 || k < start) continue;
       const int p1 = rp1[m], p2 = rp2[m];
       if (last >= 0) {
-        double d12 = TA[p1] * VA[p2] + TA[N + p1] * VA[N + p2];
-        double d21 = TA[p2] * VA[p1] + TA[N + p2] * VA[N + p1];
+        double d12 = TA[pt_ix<R>(N, 0, p1)] * VA[pt_ix<R>(N, 0, p2)] + TA[pt_ix<R>(N, 1, p1)] * VA[pt_ix<R>(N, 1, p2)];
+        double d21 = TA[pt_ix<R>(N, 0, p2)] * VA[pt_ix<R>(N, 0, p1)] + TA[pt_ix<R>(N, 1, p2)] * VA[pt_ix<R>(N, 1, p1)];
         r12[m] = r12[m] - d12;
         r21[m] = r21[m] - d21;
         o12[rb_[m]] = r12[m];
         o21[rb_[m]] = r21[m];
-        double e1 = TA[p1] * VA[p1] + TA[N + p1] * VA[N + p1];
-        double e2 = TA[p2] * VA[p2] + TA[N + p2] * VA[N + p2];
+        double e1 = TA[pt_ix<R>(N, 0, p1)] * VA[pt_ix<R>(N, 0, p1)] + TA[pt_ix<R>(N, 1, p1)] * VA[pt_ix<R>(N, 1, p1)];
+        double e2 = TA[pt_ix<R>(N, 0, p2)] * VA[pt_ix<R>(N, 0, p2)] + TA[pt_ix<R>(N, 1, p2)] * VA[pt_ix<R>(N, 1, p2)];
         r11[m] = r11[m] - e1;
         r22[m] = r22[m] - e2;
       }
@@ -559,7 +574,7 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
       s_dec[kmin >> 5] |= 1u << (kmin & 31);
       tot = tot + 1;
     }
-    double *TN = PT + (size_t)4 * N * na;  // slot of this accept: T0, T1, V0, V1
+    double *TN = PT + (size_t)4 * N * na;  // slot of this accept: T0/T1 and V0/V1 interleaved
     for (int q = tid; q < 2 * N; q += nth) {
       if (q < N) {
         // column part: current G(i,p1), G(i,p2) with pending accepts applied in order
@@ -576,13 +591,13 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
         double c2 = (double)x2;
         for (int k = 0; k < na; ++k) {
           const double *P = PT + (size_t)4 * N * k;
-          double ti0 = P[i], ti1 = P[N + i];
-          double vp10 = P[2 * N + p1], vp11 = P[3 * N + p1], vp20 = P[2 * N + p2], vp21 = P[3 * N + p2];
+          double ti0 = P[pt_ix<R>(N, 0, i)], ti1 = P[pt_ix<R>(N, 1, i)];
+          double vp10 = P[2 * N + pt_ix<R>(N, 0, p1)], vp11 = P[2 * N + pt_ix<R>(N, 1, p1)], vp20 = P[2 * N + pt_ix<R>(N, 0, p2)], vp21 = P[2 * N + pt_ix<R>(N, 1, p2)];
           c1 = c1 - (ti0 * vp10 + ti1 * vp11);
           c2 = c2 - (ti0 * vp20 + ti1 * vp21);
         }
-        TN[i] = c1 * Pi11 + c2 * Pi21;
-        TN[N + i] = c1 * Pi12 + c2 * Pi22;
+        TN[pt_ix<R>(N, 0, i)] = c1 * Pi11 + c2 * Pi21;
+        TN[pt_ix<R>(N, 1, i)] = c1 * Pi12 + c2 * Pi22;
       } else {
         // row part: current G(p1,i), G(p2,i)
         const int i = q - N;
@@ -598,8 +613,8 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
         double r2 = (double)x2;
         for (int k = 0; k < na; ++k) {
           const double *P = PT + (size_t)4 * N * k;
-          double tp10 = P[p1], tp11 = P[N + p1], tp20 = P[p2], tp21 = P[N + p2];
-          double vi0 = P[2 * N + i], vi1 = P[3 * N + i];
+          double tp10 = P[pt_ix<R>(N, 0, p1)], tp11 = P[pt_ix<R>(N, 1, p1)], tp20 = P[pt_ix<R>(N, 0, p2)], tp21 = P[pt_ix<R>(N, 1, p2)];
+          double vi0 = P[2 * N + pt_ix<R>(N, 0, i)], vi1 = P[2 * N + pt_ix<R>(N, 1, i)];
           r1 = r1 - (tp10 * vi0 + tp11 * vi1);
           r2 = r2 - (tp20 * vi0 + tp21 * vi1);
         }
@@ -613,8 +628,8 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
           v0 = v0 + D12;
           v1 = v1 + D22;
         }
-        TN[2 * N + i] = v0;
-        TN[3 * N + i] = v1;
+        TN[2 * N + pt_ix<R>(N, 0, i)] = v0;
+        TN[2 * N + pt_ix<R>(N, 1, i)] = v1;
       }
     }
     __syncthreads();
@@ -633,10 +648,10 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   for (int idx = tid; idx < na * N; idx += nth) {
     const int k = idx / N, i = idx - k * N;
     const double *P = PT + (size_t)4 * N * k;
-    Tb[(2 * k) * N + i] = (R)P[i];
-    Tb[(2 * k + 1) * N + i] = (R)P[N + i];
-    Vb[(2 * k) * N + i] = (R)P[2 * N + i];
-    Vb[(2 * k + 1) * N + i] = (R)P[3 * N + i];
+    Tb[(2 * k) * N + i] = (R)P[pt_ix<R>(N, 0, i)];
+    Tb[(2 * k + 1) * N + i] = (R)P[pt_ix<R>(N, 1, i)];
+    Vb[(2 * k) * N + i] = (R)P[2 * N + pt_ix<R>(N, 0, i)];
+    Vb[(2 * k + 1) * N + i] = (R)P[2 * N + pt_ix<R>(N, 1, i)];
   }
   for (int b = tid; b < N2; b += nth) sig_t[b] = sg[b];
   if (a.record)
