@@ -149,18 +149,20 @@ def test_parity_small_delay_depth_full_sweeps():
             _close_G(e.green(k), c.green())
 
 
-def test_mixed_precision_shadow():
+@pytest.mark.parametrize("L, T, beta, nsw", [(8, 40, 4.0, 3), (12, 20, 2.0, 1), (16, 20, 2.0, 1)])
+def test_mixed_precision_shadow(L, T, beta, nsw):
     """FP32 propagation replaying the FP64 decisions: max|G32 - G64| before each
-    recompute stays small and the observables agree."""
+    recompute stays small and the observables agree (L = 8, 12, 16: the FP32
+    flip kernels compiled for N = 64, 144, 256)."""
     import z2_gauge_zfy_amd as z
-    L, T, beta, w = 8, 40, 4.0, 10
+    w = 10
     specs = [z.ChainSpec(h=0.4 + 0.2 * k, Beta=beta, seed=12345 + 2 * 7919 * k) for k in range(4)]
     ref = z.Engine(L, L, T, w, specs, precision=z.Z2_PREC_FP64)
     mix = z.Engine(L, L, T, w, specs, precision=z.Z2_PREC_MIXED)
     for e in (ref, mix):
         e.init_fields()
         e.pre()
-    for sw in range(3):
+    for sw in range(nsw):
         d = mix.sweep_shadow(ref, True)
         print("sweep", sw, "max|dG| per chain", d, flush=True)
         assert np.all(d < 1e-4), d

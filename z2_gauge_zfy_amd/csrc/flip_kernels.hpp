@@ -108,9 +108,11 @@ __device__ __forceinline__ int pt_ix(int N, int c, int i) {
   return sizeof(R) == 8 ? 2 * i + c : c * N + i;
 }
 
-template <typename R, int MPT>
+// NC: the site count when fixed at compile time (0: a.lat.N), so that the
+// distances between the pending vectors and G's columns are constants
+template <typename R, int MPT, int NC = 0>
 __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
-  const int N = a.lat.N, N2 = a.lat.N2;
+  const int N = NC ? NC : a.lat.N, N2 = NC ? 2 * NC : a.lat.N2;
   const int chain = blockIdx.x;
   const int tid = threadIdx.x, nth = blockDim.x;
   const ChainConst *cp = a.cc + chain;
