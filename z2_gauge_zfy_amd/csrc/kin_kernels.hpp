@@ -86,12 +86,13 @@ __device__ __forceinline__ int kin_tab_entry(const LatticeDev &lat, const int8_t
 // A thread owns one bond of a group and a set of the block's columns: its
 // rotation data is read once per stage.
 // grid: (ceil(N/CB), n_chains, n_mats)
-template <typename R, int CB>
+// NC: the site count when fixed at compile time (0: a.lat.N)
+template <typename R, int CB, int NC = 0>
 __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
   R *tile = reinterpret_cast<R *>(smem_raw);                   // [CB][N]
   int *tab = reinterpret_cast<int *>(tile + (size_t)CB * a.lat.N);  // [4][N/2]
-  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
+  const int N = NC ? NC : a.lat.N, N2 = 2 * N, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
   const int chain = blockIdx.y;
   const KinMat<R> km = a.m[blockIdx.z];
   R *M = km.mat + (size_t)chain * N * N;
@@ -135,12 +136,12 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
 
 // Mat <- Mat B^{nflag} : each row is independent; a block owns RB rows.  Thread
 // tid works on row tid % RB of the block and on bonds tid / RB + j * (nt / RB).
-template <typename R, int RB>
+template <typename R, int RB, int NC = 0>
 __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
   R *tile = reinterpret_cast<R *>(smem_raw);                   // [N][RB]
   int *tab = reinterpret_cast<int *>(tile + (size_t)RB * a.lat.N);  // [4][N/2]
-  const int N = a.lat.N, N2 = a.lat.N2, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
+  const int N = NC ? NC : a.lat.N, N2 = 2 * N, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
   const int chain = blockIdx.y;
   const KinMat<R> km = a.m[blockIdx.z];
   R *M = km.mat + (size_t)chain * N * N;

@@ -867,28 +867,51 @@ static const int KCB = 8;  // columns per block, row-rotation kernel (8: best me
 static constexpr int FLIP_THREADS = 512;  // must not exceed k_flip's __launch_bounds__
 static const int KRB = 8;  // rows per block, column-rotation kernel (8: best measured of 4, 8, 16, 32)
 
-template <typename R>
-static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
+// the kinetic kernels compiled for N = 64, 144, 256 (L = 8, 12, 16), else for any N
+template <typename R, int NC>
+static int launch_rows_n(z2_ctx *c, KinArgs<R> &a, int nmat) {
   dim3 grid((c->N + KCB - 1) / KCB, c->nch, nmat);
   size_t sh = sizeof(R) * KCB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
   if (sh > 65536)
-    HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, KCB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  CHK(prof_begin(c, Z2_K_KIN_ROWS));
-  hipLaunchKernelGGL((k_kin_rows<R, KCB>), grid, dim3(256), sh, c->stream, a);
+    HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, KCB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL((k_kin_rows<R, KCB, NC>), grid, dim3(256), sh, c->stream, a);
   LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+template <typename R, int NC>
+static int launch_cols_n(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  dim3 grid((c->N + KRB - 1) / KRB, c->nch, nmat);
+  size_t sh = sizeof(R) * KRB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+  if (sh > 65536)
+    HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, KRB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL((k_kin_cols<R, KRB, NC>), grid, dim3(256), sh, c->stream, a);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+template <typename R>
+static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  CHK(prof_begin(c, Z2_K_KIN_ROWS));
+  switch (c->N) {
+    case 64: CHK((launch_rows_n<R, 64>(c, a, nmat))); break;
+    case 144: CHK((launch_rows_n<R, 144>(c, a, nmat))); break;
+    case 256: CHK((launch_rows_n<R, 256>(c, a, nmat))); break;
+    default: CHK((launch_rows_n<R, 0>(c, a, nmat)));
+  }
   CHK(prof_end(c, Z2_K_KIN_ROWS));
   return Z2_OK;
 }
 
 template <typename R>
 static int launch_cols(z2_ctx *c, KinArgs<R> &a, int nmat) {
-  dim3 grid((c->N + KRB - 1) / KRB, c->nch, nmat);
-  size_t sh = sizeof(R) * KRB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
-  if (sh > 65536)
-    HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, KRB>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   CHK(prof_begin(c, Z2_K_KIN_COLS));
-  hipLaunchKernelGGL((k_kin_cols<R, KRB>), grid, dim3(256), sh, c->stream, a);
-  LAUNCH_CHECK();
+  switch (c->N) {
+    case 64: CHK((launch_cols_n<R, 64>(c, a, nmat))); break;
+    case 144: CHK((launch_cols_n<R, 144>(c, a, nmat))); break;
+    case 256: CHK((launch_cols_n<R, 256>(c, a, nmat))); break;
+    default: CHK((launch_cols_n<R, 0>(c, a, nmat)));
+  }
   CHK(prof_end(c, Z2_K_KIN_COLS));
   return Z2_OK;
 }
