@@ -29,6 +29,9 @@ struct QrbArgs {
   const double *Ain;  // orgqr: reflectors (chain stride N*N)
   const double *tin;  // orgqr: tau
   double *Q;          // orgqr: output (chain stride N*N)
+  const double *Bq;   // Q^T B: B (chain stride N*N), read as B^T when btrans
+  double *Xq;         // Q^T B: output X (chain stride N*N), written as X^T when xtrans
+  int btrans, xtrans;
   int N, tc, nopiv;
 };
 
@@ -682,6 +685,57 @@ __global__ void __launch_bounds__(512) k_orgqr_t(QrbArgs a, double *Tws) {
 // panel p's block reflector touches rows j0.. of it, and panels that start after
 // the tile's last column leave it unchanged.  The tiles with the most panels are
 // dispatched first.  Each panel application is the same as in k_orgqr_blk.
+// X = Q^T op(B) (or its transpose) with Q held as the block reflectors of the QR
+// (T factors of every panel in Tws, from k_orgqr_t): one column tile of op(B) per
+// workgroup, kept in LDS while panel 0, 1, ... apply (I - V T^T V^T) to its rows
+// j0.. (Q^T = H_{n-1} ... H_0) -- the trailing update of the factorization,
+// without forming Q.  op(B) = B^T is read along B's rows, X^T written along X's rows.
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_qt_apply(QrbArgs a, const double *Tws) {
+  const int N = a.N, tc = a.tc, chain = blockIdx.y;
+  const int npan = (N + NB - 1) / NB;
+  const int c0 = blockIdx.x * tc, tcn = min(tc, N - c0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const double *A = a.Ain + (size_t)chain * N * N;
+  const double *B = a.Bq + (size_t)chain * N * N;
+  double *X = a.Xq + (size_t)chain * N * N;
+  extern __shared__ unsigned char smem_raw[];
+  QrbLds L = qrb_carve(smem_raw, N, NB, tc);
+  if (a.btrans) {
+    for (int e = threadIdx.x; e < 32 * N; e += blockDim.x) {  // tc <= 32
+      const int c = e & 31, r = e >> 5;
+      if (c < tcn) L.Ct[(size_t)c * L.ld + r] = B[(c0 + c) + (size_t)r * N];
+    }
+  } else {
+    qrb_tile_load<RPL>(L.Ct, L.ld, B + (size_t)c0 * N, N, nullptr, N, tcn);
+  }
+  for (int p = 0; p < npan; ++p) {
+    const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
+    __syncthreads();  // the tile is staged / the previous panel is done with Pn and Tm
+    qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);
+    const double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) L.Tm[(e % NB) + (size_t)(e / NB) * L.lw] = T[e];
+    __syncthreads();
+    QrbLds Lp = L;
+    Lp.Ct = L.Ct + j0;  // rows j0.. of the tile
+    qrb_apply_tile<NB>(Lp, m, w, tcn, 1);
+  }
+  __syncthreads();
+  if (a.xtrans) {
+    for (int e = threadIdx.x; e < 32 * N; e += blockDim.x) {
+      const int c = e & 31, r = e >> 5;
+      if (c < tcn) X[(c0 + c) + (size_t)r * N] = L.Ct[(size_t)c * L.ld + r];
+    }
+  } else {
+    for (int c = wave; c < tcn; c += nw)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        if (r < N) X[r + (size_t)(c0 + c) * N] = L.Ct[(size_t)c * L.ld + r];
+      }
+  }
+}
+
 template <int NB, int RPL>
 __global__ void __launch_bounds__(512) k_orgqr_tiles(QrbArgs a, const double *Tws) {
   const int N = a.N, tc = a.tc, chain = blockIdx.y;

@@ -1160,32 +1160,38 @@ static int launch_qrb(int orgqr, int nb, int nmat, size_t sh, hipStream_t st, co
 
 // explicit Q over many workgroups per matrix (panel T factors, then one launch per panel)
 template <int NB, int RPL>
-static int launch_orgqr_split_t(int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q, double *Tws) {
+static int launch_orgqr_split_t(int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q, double *Tws,
+                                int apply = 0) {
   const int npan = (n + NB - 1) / NB;
   if (sh > 65536) {
     HIPC(hipFuncSetAttribute((const void *)k_orgqr_t<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     HIPC(hipFuncSetAttribute((const void *)k_orgqr_tiles<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    HIPC(hipFuncSetAttribute((const void *)k_qt_apply<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   }
   hipLaunchKernelGGL((k_orgqr_t<NB, RPL>), dim3(npan, nmat), dim3(512), sh, st, q, Tws);
   LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_orgqr_tiles<NB, RPL>), dim3((n + tc - 1) / tc, nmat), dim3(512), sh, st, q,
-                     (const double *)Tws);
+  if (apply)
+    hipLaunchKernelGGL((k_qt_apply<NB, RPL>), dim3((n + tc - 1) / tc, nmat), dim3(512), sh, st, q,
+                       (const double *)Tws);
+  else
+    hipLaunchKernelGGL((k_orgqr_tiles<NB, RPL>), dim3((n + tc - 1) / tc, nmat), dim3(512), sh, st, q,
+                       (const double *)Tws);
   LAUNCH_CHECK();
   return Z2_OK;
 }
 
 template <int NB>
 static int launch_orgqr_split_nb(int rpl, int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q,
-                                 double *Tws) {
+                                 double *Tws, int apply = 0) {
   switch (rpl) {
-    case 1: return launch_orgqr_split_t<NB, 1>(n, tc, nmat, sh, st, q, Tws);
-    case 2: return launch_orgqr_split_t<NB, 2>(n, tc, nmat, sh, st, q, Tws);
-    case 3: return launch_orgqr_split_t<NB, 3>(n, tc, nmat, sh, st, q, Tws);
-    case 4: return launch_orgqr_split_t<NB, 4>(n, tc, nmat, sh, st, q, Tws);
-    case 5: return launch_orgqr_split_t<NB, 5>(n, tc, nmat, sh, st, q, Tws);
-    case 6: return launch_orgqr_split_t<NB, 6>(n, tc, nmat, sh, st, q, Tws);
-    case 7: return launch_orgqr_split_t<NB, 7>(n, tc, nmat, sh, st, q, Tws);
-    case 8: return launch_orgqr_split_t<NB, 8>(n, tc, nmat, sh, st, q, Tws);
+    case 1: return launch_orgqr_split_t<NB, 1>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 2: return launch_orgqr_split_t<NB, 2>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 3: return launch_orgqr_split_t<NB, 3>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 4: return launch_orgqr_split_t<NB, 4>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 5: return launch_orgqr_split_t<NB, 5>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 6: return launch_orgqr_split_t<NB, 6>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 7: return launch_orgqr_split_t<NB, 7>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 8: return launch_orgqr_split_t<NB, 8>(n, tc, nmat, sh, st, q, Tws, apply);
   }
   return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
 }
@@ -1309,6 +1315,38 @@ static int orgqr_n(z2_ctx *c, const double *A, const double *tau, double *Qo, in
 
 static int orgqr(z2_ctx *c, const double *A, double *Qo) { return orgqr_n(c, A, c->tau, Qo, c->N); }
 
+// X = Q^T B^T (xtrans = 0, B read transposed) or X = B^T Q = (Q^T B)^T (xtrans = 1),
+// Q from the last blocked QR of A (reflectors in A, tau): the block reflectors are
+// applied to the tiles of B directly; without the split blocked path, Q is formed
+// and multiplied
+static int apply_qt(z2_ctx *c, const double *A, const double *B, double *X, int xtrans) {
+  const int n = c->N;
+  int nb = 0, tc = 0;
+  if (c->qr_mode == Z2_QR_BLOCKED && c->ortw && qr_workspace(c, n) && qrb_config(c->lds_max, n, &nb, &tc) &&
+      tc <= 32) {
+    QrbArgs q;
+    memset(&q, 0, sizeof(q));
+    q.Ain = A;
+    q.tin = c->tau;
+    q.Bq = B;
+    q.Xq = X;
+    q.btrans = xtrans ? 0 : 1;
+    q.xtrans = xtrans;
+    q.N = n;
+    q.tc = tc;
+    CHK(prof_begin(c, Z2_K_ORGQR));
+    const int rpl = (n + 63) / 64;
+    const size_t sh = qrb_lds_bytes(n, nb, tc);
+    if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1));
+    else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1));
+    CHK(prof_end(c, Z2_K_ORGQR));
+    return Z2_OK;
+  }
+  CHK(orgqr(c, A, c->Q));
+  if (xtrans) return gemm(c, B, 1, c->Q, 0, X, 0, 0);
+  return gemm(c, c->Q, 1, B, 1, X, 0, 0);
+}
+
 // n x n triangular solves for nb matrices; the right-hand-side block shrinks so that
 // (cbt + 1) n doubles fit the LDS of one workgroup
 template <int MJ>
@@ -1422,15 +1460,14 @@ static int stab_green(z2_ctx *c, int nt) {
     LAUNCH_CHECK();
   }
   CHK(qrp(c, c->M, c->Dtmp));
-  CHK(orgqr(c, c->M, c->Q));
   if (nt < c->T / 2 + 1) {
-    CHK(gemm(c, c->Q, 1, c->UUR, 1, c->X));
+    CHK(apply_qt(c, c->M, c->UUR, c->X, 0));  // X = Q^T UUR^T
     CHK(elem(c, c->X, c->X, 8, c->Dtmp));
     CHK(trsm(c, c->M, c->X, 0));
     CHK(elem(c, c->X, c->Y, 6, nullptr, c->ipvt));
     CHK(gemm(c, c->UUL, 1, c->Y, 0, c->Gn));
   } else {
-    CHK(gemm(c, c->UUL, 1, c->Q, 0, c->X));
+    CHK(apply_qt(c, c->M, c->UUL, c->X, 1));  // X = UUL^T Q
     CHK(elem(c, c->X, c->X, 9, c->Dtmp));
     CHK(trsm(c, c->M, c->X, 1));
     CHK(elem(c, c->X, c->Y, 7, nullptr, c->ipvt));
