@@ -1,7 +1,13 @@
 ! Fortran API surface of the reference (module and procedure names of
-! src/model.f90, src/stabilization.f90, src/obser_equal.f90, src/local_sweep.f90)
-! over the gfx950 engine.  Propagator and WrapList live on the GPU; the Fortran
-! types keep the reference's argument lists and carry only the chain index.
+! src/model.f90, src/process_matrix.f90, src/multiply.f90, src/localK.f90,
+! src/stabilization.f90, src/obser_equal.f90, src/obser_tau.f90,
+! src/dynamics.f90, src/local_sweep.f90) over the gfx950 engine.  Propagator,
+! PropGreen and WrapList live on the GPU; the Fortran types keep the
+! reference's argument lists.  The two spin copies go through identical
+! operations in the reference (src/localK.f90:85-118, src/multiply.f90:8-130),
+! so the engine keeps one copy: procedures called for a spin-down object (the
+! second object of each type made, as the reference makes PropU then PropD)
+! do nothing, and pair procedures (PropU, PropD) act once.
 module DQMC_Model_mod
     use iso_c_binding
     use z2dqmc_c
@@ -17,6 +23,10 @@ module DQMC_Model_mod
     integer :: n_chains = 1
     integer :: max_chains = 0
     type(c_ptr) :: ctx = c_null_ptr
+    ! the parameters and chains the context was made from (for a second context of
+    ! the same chains)
+    type(z2_params) :: ctx_params
+    type(z2_chain_desc), allocatable :: ctx_chains(:)
 contains
     ! read_input + Params_set (src/calc_basic.f90:50-102)
     subroutine read_input()
@@ -74,6 +84,8 @@ contains
         p = z2_params(int(Nlx, c_int32_t), int(Nly, c_int32_t), int(Ltrot, c_int32_t), &
                       int(Nwrap, c_int32_t), int(Nsweep, c_int32_t), int(precision_mode, c_int32_t))
         call z2_check(z2_ctx_create(p, int(n_chains, c_int32_t), d, ctx), "z2_ctx_create")
+        ctx_params = p
+        ctx_chains = d
         if (conf0 == 0) then
             call z2_check(z2_init_fields(ctx), "z2_init_fields")
         else
@@ -157,26 +169,47 @@ end module DQMC_Model_mod
 
 module ProcessMatrix
     implicit none
-    ! device-resident: the Fortran objects only name the chain they stand for
+    ! device-resident (src/process_matrix.f90:6-44): the Fortran objects carry the
+    ! drift figures and whether they stand for the spin-down copy
     type :: Propagator
         integer :: chain = 0
+        logical :: down = .false.
         real(kind=8) :: Xmaxm = 0.d0, Xmeanm = 0.d0
     contains
         procedure :: make => Prop_make
     end type Propagator
+    type :: PropGreen
+        logical :: down = .false.
+        real(kind=8) :: Xmaxm(3) = 0.d0, Xmeanm(3) = 0.d0
+    contains
+        procedure :: make => Propgr_make
+    end type PropGreen
     type :: WrapList
         integer :: chain = 0
+        logical :: down = .false.
     contains
         procedure :: make => Wrlist_make
     end type WrapList
+    integer, save, private :: n_prop = 0, n_propgr = 0, n_wrlist = 0
 contains
     subroutine Prop_make(this)
         class(Propagator), intent(inout) :: this
+        n_prop = n_prop + 1
+        this%down = mod(n_prop, 2) == 0
         this%Xmaxm = 0.d0
         this%Xmeanm = 0.d0
     end subroutine Prop_make
+    subroutine Propgr_make(this)
+        class(PropGreen), intent(inout) :: this
+        n_propgr = n_propgr + 1
+        this%down = mod(n_propgr, 2) == 0
+        this%Xmaxm = 0.d0
+        this%Xmeanm = 0.d0
+    end subroutine Propgr_make
     subroutine Wrlist_make(this)
         class(WrapList), intent(inout) :: this
+        n_wrlist = n_wrlist + 1
+        this%down = mod(n_wrlist, 2) == 0
         this%chain = 0
     end subroutine Wrlist_make
 end module ProcessMatrix
@@ -197,6 +230,7 @@ contains
         class(Propagator), intent(inout) :: Prop
         class(WrapList), intent(inout) :: WrList
         integer, intent(in) :: nt
+        if (Prop%down) return
         call z2_check(z2_wrap_pre(ctx, int(nt, c_int32_t)), "Wrap_pre")
     end subroutine Wrap_pre
     subroutine Wrap_L(Prop, WrList, nt, flag)
@@ -205,6 +239,7 @@ contains
         integer, intent(in) :: nt
         character(len=*), optional, intent(in) :: flag
         integer(c_int32_t) :: f
+        if (Prop%down) return
         f = 0
         if (present(flag)) f = 1
         call z2_check(z2_wrap_L(ctx, int(nt, c_int32_t), f), "Wrap_L")
@@ -215,16 +250,27 @@ contains
         integer, intent(in) :: nt
         character(len=*), optional, intent(in) :: flag
         integer(c_int32_t) :: f
+        if (Prop%down) return
         f = 0
         if (present(flag)) f = 1
         call z2_check(z2_wrap_R(ctx, int(nt, c_int32_t), f), "Wrap_R")
     end subroutine Wrap_R
+    ! src/stabilization.f90:478-514 on the time-displaced state
+    subroutine Wrap_tau(Prop, PropGr, WrList, nt)
+        class(Propagator), intent(inout) :: Prop
+        class(PropGreen), intent(inout) :: PropGr
+        class(WrapList), intent(in) :: WrList
+        integer, intent(in) :: nt
+        if (Prop%down) return
+        call z2_check(z2_wrap_tau(ctx, int(nt, c_int32_t)), "Wrap_tau")
+    end subroutine Wrap_tau
 end module Stabilize_mod
 
 module ObserEqual_mod
     use iso_c_binding
     use z2dqmc_c
     use DQMC_Model_mod, only: ctx, n_chains
+    use ProcessMatrix
     implicit none
     ! src/obser_equal.f90:7-20 (chain-averaged, as after the per-bin MPI_REDUCE)
     type, public :: ObserEqual
@@ -232,26 +278,47 @@ module ObserEqual_mod
     contains
         procedure :: make => Obs_equal_make
         procedure :: reset => Obs_equal_reset
+        procedure :: calc => Obs_equal_calc
         procedure :: ave => Obs_equal_ave
+        procedure :: collect => Obs_equal_collect
     end type ObserEqual
 contains
     subroutine Obs_equal_make(this)
         class(ObserEqual), intent(inout) :: this
-        call this%reset()
+        call obs_equal_zero(this)
     end subroutine Obs_equal_make
-    subroutine Obs_equal_reset(this)
+    subroutine obs_equal_zero(this)
         class(ObserEqual), intent(inout) :: this
         this%flux_avg = 0.d0; this%chi_flux = 0.d0; this%pair_sc_q0 = 0.d0
         this%cdw_pi = 0.d0; this%cdw_shift = 0.d0; this%cdw_ratio = 0.d0; this%density = 0.d0
+    end subroutine obs_equal_zero
+    ! src/obser_equal.f90:33-48
+    subroutine Obs_equal_reset(this)
+        class(ObserEqual), intent(inout) :: this
+        call obs_equal_zero(this)
+        if (c_associated(ctx)) call z2_check(z2_obs_reset(ctx), "Obs_equal%reset")
     end subroutine Obs_equal_reset
-    ! Obs_equal%ave (src/obser_equal.f90:50-71) + the per-bin rank average of
-    ! m_process_obs_equal (src/fourier_trans.f90:280-315)
+    ! src/obser_equal.f90:73-149, all chains at slice ntau
+    subroutine Obs_equal_calc(this, PropU, PropD, ntau)
+        class(ObserEqual), intent(inout) :: this
+        class(Propagator), intent(in) :: PropU, PropD
+        integer, intent(in) :: ntau
+        call z2_check(z2_obs_equal_calc(ctx, int(ntau, c_int32_t)), "Obs_equal%calc")
+    end subroutine Obs_equal_calc
+    ! Obs_equal%ave (src/obser_equal.f90:50-71) on the engine, then the chain average
     subroutine Obs_equal_ave(this, Nobs)
         class(ObserEqual), intent(inout) :: this
         integer, intent(in) :: Nobs
+        call z2_check(z2_obs_equal_ave(ctx, int(Nobs, c_int32_t)), "Obs_equal%ave")
+        call this%collect()
+    end subroutine Obs_equal_ave
+    ! the per-bin rank average of m_process_obs_equal (src/fourier_trans.f90:280-315)
+    ! over the chains of the per-chain averages the engine holds
+    subroutine Obs_equal_collect(this)
+        class(ObserEqual), intent(inout) :: this
         real(c_double) :: o(7)
         integer :: k
-        call this%reset()
+        call obs_equal_zero(this)
         do k = 0, n_chains - 1
             call z2_check(z2_get_obs_equal(ctx, int(k, c_int32_t), o), "z2_get_obs_equal")
             this%flux_avg = this%flux_avg + o(1)
@@ -269,14 +336,14 @@ contains
         this%cdw_shift = this%cdw_shift / dble(n_chains)
         this%cdw_ratio = this%cdw_ratio / dble(n_chains)
         this%density = this%density / dble(n_chains)
-        if (Nobs < 0) continue
-    end subroutine Obs_equal_ave
+    end subroutine Obs_equal_collect
 end module ObserEqual_mod
 
 module ObserTau_mod
     use iso_c_binding
     use z2dqmc_c
     use DQMC_Model_mod, only: ctx, n_chains, Lq, Ltrot
+    use ProcessMatrix
     implicit none
     ! src/obser_tau.f90:7-16: Lq x Ltrot correlators at displacement imj and slice
     type, public :: ObserTau
@@ -285,31 +352,54 @@ module ObserTau_mod
     contains
         procedure :: make => Obs_tau_make
         procedure :: reset => Obs_tau_reset
+        procedure :: calc => Obs_tau_calc
         procedure :: ave => Obs_tau_ave
+        procedure :: collect => Obs_tau_collect
     end type ObserTau
+    ! the engine's accumulators exist (z2_enable_tau has run)
+    logical :: tau_on_device = .false.
 contains
     subroutine Obs_tau_make(this)
         class(ObserTau), intent(inout) :: this
         allocate(this%spin_tau(Lq, Ltrot), this%pair_s_tau(Lq, Ltrot), this%pair_d_tau(Lq, Ltrot))
         allocate(this%charge_s_tau(Lq, Ltrot), this%charge_d_tau(Lq, Ltrot), this%single_tau(Lq, Ltrot))
         allocate(this%curxx_tau(Lq, Ltrot))
-        call this%reset()
+        call obs_tau_zero(this)
     end subroutine Obs_tau_make
-    subroutine Obs_tau_reset(this)
+    subroutine obs_tau_zero(this)
         class(ObserTau), intent(inout) :: this
         this%spin_tau = (0.d0, 0.d0); this%pair_s_tau = (0.d0, 0.d0); this%pair_d_tau = (0.d0, 0.d0)
         this%charge_s_tau = (0.d0, 0.d0); this%charge_d_tau = (0.d0, 0.d0); this%single_tau = (0.d0, 0.d0)
         this%curxx_tau = (0.d0, 0.d0)
+    end subroutine obs_tau_zero
+    ! src/obser_tau.f90:26-35
+    subroutine Obs_tau_reset(this)
+        class(ObserTau), intent(inout) :: this
+        call obs_tau_zero(this)
+        if (tau_on_device) call z2_check(z2_obs_tau_reset(ctx), "Obs_tau%reset")
     end subroutine Obs_tau_reset
-    ! Obs_tau%ave (src/obser_tau.f90:37-47, done by the engine per chain) + the rank
-    ! average of m_process_obs_tau (src/fourier_trans.f90:451-488) over the chains
+    ! src/obser_tau.f90:56-133, all chains at slice ntau
+    subroutine Obs_tau_calc(this, PropGrU, PropGrD, ntau)
+        class(ObserTau), intent(inout) :: this
+        class(PropGreen), intent(in) :: PropGrU, PropGrD
+        integer, intent(in) :: ntau
+        call z2_check(z2_obs_tau_calc(ctx, int(ntau, c_int32_t)), "Obs_tau%calc")
+    end subroutine Obs_tau_calc
+    ! Obs_tau%ave (src/obser_tau.f90:37-47) on the engine, then the chain average
     subroutine Obs_tau_ave(this, Nobst)
         class(ObserTau), intent(inout) :: this
         integer, intent(in) :: Nobst
+        call z2_check(z2_obs_tau_ave(ctx, int(Nobst, c_int32_t)), "Obs_tau%ave")
+        call this%collect()
+    end subroutine Obs_tau_ave
+    ! the rank average of m_process_obs_tau (src/fourier_trans.f90:451-488) over the
+    ! chains of the per-chain averages the engine holds
+    subroutine Obs_tau_collect(this)
+        class(ObserTau), intent(inout) :: this
         real(c_double), allocatable :: o(:, :, :)
         integer :: k
         allocate(o(Lq, Ltrot, 7))
-        call this%reset()
+        call obs_tau_zero(this)
         do k = 0, n_chains - 1
             call z2_check(z2_get_obs_tau(ctx, int(k, c_int32_t), o), "z2_get_obs_tau")
             this%spin_tau = this%spin_tau + o(:, :, 1)
@@ -327,8 +417,7 @@ contains
         this%charge_d_tau = this%charge_d_tau / dcmplx(dble(n_chains), 0.d0)
         this%single_tau = this%single_tau / dcmplx(dble(n_chains), 0.d0)
         this%curxx_tau = this%curxx_tau / dcmplx(dble(n_chains), 0.d0)
-        if (Nobst < 0) continue
-    end subroutine Obs_tau_ave
+    end subroutine Obs_tau_collect
 end module ObserTau_mod
 
 ! Output of the time-displaced correlators (m_write_obs_tau, src/fourier_trans.f90:316-449):
@@ -551,6 +640,141 @@ contains
     end subroutine Fourier_prtau
 end module FourierTrans_mod
 
+! src/multiply.f90:8-130: which matrices get B or B^{-1} from which side at each
+! slice; all chains at slice nt (sigma and lambda live on the GPU: the array
+! arguments are kept for the reference's signatures)
+module Multiply_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod, only: ctx, Lq, Ltrot
+    use ProcessMatrix
+    implicit none
+contains
+    subroutine propK_pre(PropU, PropD, sigma, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        real(kind=8), dimension(2*Lq, Ltrot), intent(in) :: sigma
+        integer, intent(in) :: nt
+        call z2_check(z2_prop_K_pre(ctx, int(nt, c_int32_t)), "propK_pre")
+    end subroutine propK_pre
+    subroutine propT_pre(PropU, PropD, lambda, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        real(kind=8), dimension(Lq), intent(in) :: lambda
+        integer, intent(in) :: nt
+        call z2_check(z2_prop_T_pre(ctx, int(nt, c_int32_t)), "propT_pre")
+    end subroutine propT_pre
+    subroutine propT_L(PropU, PropD, lambda, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        real(kind=8), dimension(Lq), intent(in) :: lambda
+        integer, intent(in) :: nt
+        call z2_check(z2_prop_T_L(ctx, int(nt, c_int32_t)), "propT_L")
+    end subroutine propT_L
+    subroutine propT_R(PropU, PropD, lambda, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        real(kind=8), dimension(Lq), intent(in) :: lambda
+        integer, intent(in) :: nt
+        call z2_check(z2_prop_T_R(ctx, int(nt, c_int32_t)), "propT_R")
+    end subroutine propT_R
+    subroutine propgrK_R(PropU, PropD, PropgrU, PropgrD, sigma, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(PropGreen), intent(inout) :: PropgrU, PropgrD
+        real(kind=8), dimension(2*Lq, Ltrot), intent(in) :: sigma
+        integer, intent(in) :: nt
+        call z2_check(z2_propgr_K_R(ctx, int(nt, c_int32_t)), "propgrK_R")
+    end subroutine propgrK_R
+    subroutine propgrT_R(PropU, PropD, PropgrU, PropgrD, lambda, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(PropGreen), intent(inout) :: PropgrU, PropgrD
+        real(kind=8), dimension(Lq), intent(in) :: lambda
+        integer, intent(in) :: nt
+        call z2_check(z2_propgr_T_R(ctx, int(nt, c_int32_t)), "propgrT_R")
+    end subroutine propgrT_R
+end module Multiply_mod
+
+! src/localK.f90:20-43, 347-398: the link-flip sweep of one slice and its wraps,
+! all chains; each chain's ranf seed lives on the GPU (LocalK_prop_L/R leave the
+! iseed argument as it is; Local_sweep returns chain 0's)
+module LocalK_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod, only: ctx
+    use ProcessMatrix
+    implicit none
+contains
+    subroutine LocalK_init()
+    end subroutine LocalK_init
+    subroutine LocalK_clear()
+    end subroutine LocalK_clear
+    ! Acc_Kl%reset: the engine's acceptance counters are cumulative (z2_get_counters)
+    subroutine LocalK_reset()
+    end subroutine LocalK_reset
+    subroutine LocalK_prop_L(PropU, PropD, iseed, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        integer, intent(inout) :: iseed
+        integer, intent(in) :: nt
+        call z2_check(z2_localK_prop_L(ctx, int(nt, c_int32_t)), "LocalK_prop_L")
+    end subroutine LocalK_prop_L
+    subroutine LocalK_prop_R(PropU, PropD, iseed, nt)
+        class(Propagator), intent(inout) :: PropU, PropD
+        integer, intent(inout) :: iseed
+        integer, intent(in) :: nt
+        call z2_check(z2_localK_prop_R(ctx, int(nt, c_int32_t)), "LocalK_prop_R")
+    end subroutine LocalK_prop_R
+end module LocalK_mod
+
+! src/dynamics.f90:1-95: the time-displaced Green's functions propagated over all
+! slices from the start of a right pass, re-stabilised by Wrap_tau
+module Dynamics_mod
+    use iso_c_binding
+    use z2dqmc_c
+    use DQMC_Model_mod, only: ctx, n_chains, Ltrot, Nwrap, RT, Lq
+    use ProcessMatrix
+    use Stabilize_mod
+    use Multiply_mod
+    use ObserTau_mod
+    implicit none
+    type :: Dynamics
+    contains
+        procedure, nopass :: init => Dyn_init
+        procedure, nopass :: clear => Dyn_clear
+        procedure, nopass :: reset => Dyn_reset
+        procedure, nopass :: sweep_R => Dyn_sweep_R
+    end type Dynamics
+    type(Propagator), allocatable :: Prop_dU, Prop_dD
+    type(PropGreen), allocatable :: PropGrU, PropGrD
+    real(kind=8), allocatable :: sig_arg(:, :), lam_arg(:)
+    private :: Prop_dU, Prop_dD, PropGrU, PropGrD, sig_arg, lam_arg
+contains
+    subroutine Dyn_init()
+        allocate(Prop_dU); call Prop_dU%make()
+        allocate(Prop_dD); call Prop_dD%make()
+        allocate(PropGrU); call PropGrU%make()
+        allocate(PropGrD); call PropGrD%make()
+        allocate(sig_arg(2 * Lq, Ltrot), lam_arg(Lq))
+        sig_arg = 0.d0; lam_arg = 0.d0
+    end subroutine Dyn_init
+    subroutine Dyn_clear()
+        deallocate(Prop_dU, Prop_dD, PropGrU, PropGrD, sig_arg, lam_arg)
+    end subroutine Dyn_clear
+    ! src/dynamics.f90:41-48
+    subroutine Dyn_reset(PropU, PropD)
+        class(Propagator), intent(in) :: PropU, PropD
+        call z2_check(z2_dyn_reset(ctx), "Dyn%reset")
+    end subroutine Dyn_reset
+    ! src/dynamics.f90:50-62
+    subroutine Dyn_sweep_R(Obs, WrU, WrD)
+        class(ObserTau), intent(inout) :: Obs
+        class(WrapList), intent(in) :: WrU, WrD
+        integer :: nt
+        do nt = 1, Ltrot
+            call Obs%calc(PropGrU, PropGrD, nt)
+            if (RT > 0.d0) call propgrK_R(Prop_dU, Prop_dD, PropGrU, PropGrD, sig_arg, nt)
+            call propgrT_R(Prop_dU, Prop_dD, PropGrU, PropGrD, lam_arg, nt)
+            if (mod(nt, Nwrap) == 0) call Wrap_tau(Prop_dU, PropGrU, WrU, nt)
+            if (mod(nt, Nwrap) == 0) call Wrap_tau(Prop_dD, PropGrD, WrD, nt)
+        enddo
+    end subroutine Dyn_sweep_R
+end module Dynamics_mod
+
 module LocalSweep_mod
     use iso_c_binding
     use z2dqmc_c
@@ -558,6 +782,10 @@ module LocalSweep_mod
     use ProcessMatrix
     use ObserEqual_mod
     use ObserTau_mod
+    use Stabilize_mod
+    use Multiply_mod
+    use LocalK_mod
+    use Dynamics_mod
     implicit none
     type :: LocalSweep
     contains
@@ -571,6 +799,12 @@ module LocalSweep_mod
     end type LocalSweep
     type(ObserEqual), allocatable :: Obs_equal
     type(ObserTau), allocatable :: Obs_tau
+    type(Dynamics) :: Dyn
+    ! .false.: one batched engine call per pre / sweep (z2_pre, z2_sweep_tau);
+    ! .true.: the reference's procedure sequence (src/local_sweep.f90:87-195), one
+    ! engine call per Multiply_mod / LocalK_mod / Stabilize_mod / Obs procedure
+    logical :: fine_grained = .false.
+    real(kind=8), allocatable :: sig_arg(:, :), lam_arg(:)
 contains
     subroutine Local_sweep_init(this)
         class(LocalSweep), intent(inout) :: this
@@ -580,19 +814,42 @@ contains
             allocate(Obs_tau)
             call Obs_tau%make()
             call z2_check(z2_enable_tau(ctx), "z2_enable_tau")
+            tau_on_device = .true.
+            call Dyn%init()
         endif
+        allocate(sig_arg(2 * Lq, Ltrot), lam_arg(Lq))
+        sig_arg = 0.d0; lam_arg = 0.d0
     end subroutine Local_sweep_init
     subroutine Local_sweep_clear(this)
         class(LocalSweep), intent(inout) :: this
         deallocate(Obs_equal)
-        if (allocated(Obs_tau)) deallocate(Obs_tau)
+        if (allocated(Obs_tau)) then
+            deallocate(Obs_tau)
+            call Dyn%clear()
+        endif
+        deallocate(sig_arg, lam_arg)
     end subroutine Local_sweep_clear
     ! src/local_sweep.f90:87-104
     subroutine Local_sweep_pre(this, PropU, PropD, WrU, WrD)
         class(LocalSweep), intent(inout) :: this
         class(Propagator), intent(inout) :: PropU, PropD
         class(WrapList), intent(inout) :: WrU, WrD
-        call z2_check(z2_pre(ctx), "z2_pre")
+        integer :: nt
+        if (.not. fine_grained) then
+            call z2_check(z2_pre(ctx), "z2_pre")
+            return
+        endif
+        call z2_check(z2_sweep_reset(ctx, 0_c_int32_t), "Local_sweep_reset")
+        call Wrap_pre(PropU, WrU, 0)
+        call Wrap_pre(PropD, WrD, 0)
+        do nt = 1, Ltrot
+            if (RT > 0.d0) call propK_pre(PropU, PropD, sig_arg, nt)
+            call propT_pre(PropU, PropD, lam_arg, nt)
+            if (mod(nt, Nwrap) == 0) then
+                call Wrap_pre(PropU, WrU, nt)
+                call Wrap_pre(PropD, WrD, nt)
+            endif
+        enddo
     end subroutine Local_sweep_pre
     ! Local_sweep_therm (src/local_sweep.f90:73-85), all chains
     subroutine Local_sweep_therm(this, iseed)
@@ -603,8 +860,53 @@ contains
         call z2_check(z2_get_seed(ctx, 0_c_int32_t, s), "z2_get_seed")
         iseed = s
     end subroutine Local_sweep_therm
-    ! src/local_sweep.f90:152-195 (all chains, each in its own direction: the
-    ! engine keeps every chain's is_beta; is_beta and iseed return chain 0's)
+    ! src/local_sweep.f90:106-124
+    subroutine Local_sweep_L(PropU, PropD, WrU, WrD, iseed, Nobs)
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(WrapList), intent(inout) :: WrU, WrD
+        integer, intent(inout) :: iseed, Nobs
+        integer :: nt
+        do nt = Ltrot, 1, -1
+            if (mod(nt, Nwrap) == 0) then
+                call Wrap_L(PropU, WrU, nt, "S")
+                call Wrap_L(PropD, WrD, nt, "S")
+            endif
+            call Obs_equal%calc(PropU, PropD, nt)
+            Nobs = Nobs + 1
+            call propT_L(PropU, PropD, lam_arg, nt)
+            call LocalK_prop_L(PropU, PropD, iseed, nt)
+        enddo
+        call Wrap_L(PropU, WrU, 0, "S")
+        call Wrap_L(PropD, WrD, 0, "S")
+    end subroutine Local_sweep_L
+    ! src/local_sweep.f90:126-150
+    subroutine Local_sweep_R(PropU, PropD, WrU, WrD, iseed, toggle, Nobs, Nobst)
+        class(Propagator), intent(inout) :: PropU, PropD
+        class(WrapList), intent(inout) :: WrU, WrD
+        logical, intent(in) :: toggle
+        integer, intent(inout) :: iseed, Nobs, Nobst
+        integer :: nt
+        if (toggle) then
+            call Dyn%reset(PropU, PropD)
+            call Dyn%sweep_R(Obs_tau, WrU, WrD)
+            Nobst = Nobst + 1
+        endif
+        call Wrap_R(PropU, WrU, 0, "S")
+        call Wrap_R(PropD, WrD, 0, "S")
+        do nt = 1, Ltrot
+            call LocalK_prop_R(PropU, PropD, iseed, nt)
+            call propT_R(PropU, PropD, lam_arg, nt)
+            if (mod(nt, Nwrap) == 0) then
+                call Wrap_R(PropU, WrU, nt, "S")
+                call Wrap_R(PropD, WrD, nt, "S")
+            endif
+            call Obs_equal%calc(PropU, PropD, nt)
+            Nobs = Nobs + 1
+        enddo
+    end subroutine Local_sweep_R
+    ! src/local_sweep.f90:152-195 (all chains; batched: each chain in its own
+    ! direction, the engine keeps every chain's is_beta; fine-grained: all chains in
+    ! the direction is_beta; is_beta and iseed return chain 0's)
     subroutine Local_sweep(this, PropU, PropD, WrU, WrD, iseed, is_beta, toggle)
         class(LocalSweep), intent(inout) :: this
         class(Propagator), intent(inout) :: PropU, PropD
@@ -613,23 +915,46 @@ contains
         logical, intent(inout) :: is_beta
         logical, intent(in) :: toggle
         integer(c_int32_t) :: b, s
+        integer :: Nobs, Nobst, nsw, k
         real(c_double) :: xmax, xmean
-        if (toggle) then
-            ! with the time-displaced measurement (Dyn%sweep_R at every right pass)
-            call z2_check(z2_sweep_tau(ctx, Z2_BETA_PER_CHAIN, 1_c_int32_t), "z2_sweep_tau")
-            call Obs_tau%ave(Nsweep)
+        if (fine_grained) then
+            do k = 1, n_chains - 1
+                call z2_check(z2_get_is_beta(ctx, int(k, c_int32_t), b), "z2_get_is_beta")
+                if ((b /= 0) .neqv. is_beta) then
+                    write(6, *) "fine-grained sweep: chains in different directions (use the batched sweep)"
+                    stop 1
+                endif
+            enddo
+            call z2_check(z2_sweep_reset(ctx, merge(1_c_int32_t, 0_c_int32_t, toggle)), "Local_sweep_reset")
+            Nobs = 0; Nobst = 0
+            do nsw = 1, Nsweep
+                if (is_beta) then
+                    call Local_sweep_L(PropU, PropD, WrU, WrD, iseed, Nobs)
+                    call Local_sweep_R(PropU, PropD, WrU, WrD, iseed, toggle, Nobs, Nobst)
+                else
+                    call Local_sweep_R(PropU, PropD, WrU, WrD, iseed, toggle, Nobs, Nobst)
+                    call Local_sweep_L(PropU, PropD, WrU, WrD, iseed, Nobs)
+                endif
+            enddo
+            call Obs_equal%ave(Nobs)
+            if (toggle) call Obs_tau%ave(Nobst)
         else
-            call z2_check(z2_sweep(ctx, Z2_BETA_PER_CHAIN), "z2_sweep")
+            if (toggle) then
+                ! with the time-displaced measurement (Dyn%sweep_R at every right pass)
+                call z2_check(z2_sweep_tau(ctx, Z2_BETA_PER_CHAIN, 1_c_int32_t), "z2_sweep_tau")
+                call Obs_tau%collect()
+            else
+                call z2_check(z2_sweep(ctx, Z2_BETA_PER_CHAIN), "z2_sweep")
+            endif
+            call Obs_equal%collect()
         endif
         call z2_check(z2_get_is_beta(ctx, 0_c_int32_t, b), "z2_get_is_beta")
         is_beta = b /= 0
-        call Obs_equal%ave(2 * Ltrot * Nsweep)
         call z2_check(z2_get_seed(ctx, 0_c_int32_t, s), "z2_get_seed")
         iseed = s
         call z2_check(z2_get_drift(ctx, 0_c_int32_t, xmax, xmean), "z2_get_drift")
         PropU%Xmaxm = xmax; PropU%Xmeanm = xmean
         PropD%Xmaxm = xmax; PropD%Xmeanm = xmean
-        if (toggle) continue
     end subroutine Local_sweep
     ! acceptance over all chains (src/local_sweep.f90:197-213)
     subroutine Local_control_print(toggle)

@@ -190,6 +190,111 @@ module z2dqmc_c
             real(c_double), intent(out) :: size_cluster
             integer(c_int) :: rc
         end function
+        function z2_prop_K_pre(ctx, nt) bind(C, name="z2_prop_K_pre") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_prop_T_pre(ctx, nt) bind(C, name="z2_prop_T_pre") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_prop_T_L(ctx, nt) bind(C, name="z2_prop_T_L") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_prop_T_R(ctx, nt) bind(C, name="z2_prop_T_R") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_propgr_K_R(ctx, nt) bind(C, name="z2_propgr_K_R") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_propgr_T_R(ctx, nt) bind(C, name="z2_propgr_T_R") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_localK_prop_L(ctx, nt) bind(C, name="z2_localK_prop_L") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_localK_prop_R(ctx, nt) bind(C, name="z2_localK_prop_R") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_obs_tau_calc(ctx, nt) bind(C, name="z2_obs_tau_calc") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_wrap_tau(ctx, nt) bind(C, name="z2_wrap_tau") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nt
+            integer(c_int) :: rc
+        end function
+        function z2_obs_equal_ave(ctx, nobs) bind(C, name="z2_obs_equal_ave") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nobs
+            integer(c_int) :: rc
+        end function
+        function z2_obs_tau_ave(ctx, nobst) bind(C, name="z2_obs_tau_ave") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: nobst
+            integer(c_int) :: rc
+        end function
+        function z2_sweep_reset(ctx, toggle) bind(C, name="z2_sweep_reset") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: toggle
+            integer(c_int) :: rc
+        end function
+        function z2_obs_reset(ctx) bind(C, name="z2_obs_reset") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_dyn_reset(ctx) bind(C, name="z2_dyn_reset") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_obs_tau_reset(ctx) bind(C, name="z2_obs_tau_reset") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_sync(ctx) bind(C, name="z2_sync") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int) :: rc
+        end function
+        function z2_get_dyn_drift(ctx, chain, xmax, xmean) bind(C, name="z2_get_dyn_drift") result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain
+            real(c_double), intent(out) :: xmax(3), xmean(3)
+            integer(c_int) :: rc
+        end function
         function z2_therm(ctx) bind(C, name="z2_therm") result(rc)
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

@@ -129,6 +129,10 @@ int z2_get_dyn_drift(z2_ctx *ctx, int32_t chain, double xmax[3], double xmean[3]
  * max |G_ctx - G_ref| over the propagated Green's functions just before each
  * recompute of the sweep. */
 int z2_sweep_shadow(z2_ctx *ctx, z2_ctx *ref, int32_t is_beta, double *maxdiff);
+/* the same with toggle: the time-displaced measurement (Dyn%reset + Dyn%sweep_R)
+ * at the start of the right pass on both contexts, accumulators normalised as
+ * in z2_sweep_tau (needs z2_enable_tau on both) */
+int z2_sweep_shadow_tau(z2_ctx *ctx, z2_ctx *ref, int32_t is_beta, int32_t toggle, double *maxdiff);
 
 /* fine-grained steps, all chains at slice nt (same semantics as the reference
  * routines named): */
@@ -139,6 +143,31 @@ int z2_slice_L(z2_ctx *ctx, int32_t nt);  /* propT_L + LocalK_prop_L  src/multip
 int z2_slice_R(z2_ctx *ctx, int32_t nt);  /* LocalK_prop_R + propT_R  src/localK.f90:375-398, src/multiply.f90:95-108 */
 int z2_obs_equal_calc(z2_ctx *ctx, int32_t nt);  /* Obs_equal%calc src/obser_equal.f90:73-149 */
 int z2_obs_reset(z2_ctx *ctx);                    /* Obs_equal%reset src/obser_equal.f90:33-48 */
+int z2_obs_equal_ave(z2_ctx *ctx, int32_t nobs);  /* Obs_equal%ave(Nobs) src/obser_equal.f90:50-71 */
+/* Local_sweep_reset (src/local_sweep.f90:55-71): Obs_equal%reset, with toggle
+ * Obs_tau%reset, and the field copied along tau for chains with |h| <= 1e-10 */
+int z2_sweep_reset(z2_ctx *ctx, int32_t toggle);
+
+/* Multiply_mod (src/multiply.f90), one procedure per call, all chains at slice nt.
+ * The LocalK_prop_L/R pair below plus these compose a sweep exactly as the
+ * reference's Local_sweep_L/R do (src/local_sweep.f90:106-150), with the same
+ * arithmetic as z2_sweep. */
+int z2_prop_K_pre(z2_ctx *ctx, int32_t nt);  /* propK_pre  src/multiply.f90:8-15: UUR <- B UUR */
+int z2_prop_T_pre(z2_ctx *ctx, int32_t nt);  /* propT_pre  src/multiply.f90:65-77: UUR <- UUR e^{dtau mu} */
+int z2_prop_T_L(z2_ctx *ctx, int32_t nt);    /* propT_L    src/multiply.f90:79-93: G <- G e^{+} e^{-}, UUL <- UUL e^{+} */
+int z2_prop_T_R(z2_ctx *ctx, int32_t nt);    /* propT_R    src/multiply.f90:95-108: G <- G e^{+} e^{-}, UUR <- UUR e^{+} */
+/* time-displaced state (after z2_enable_tau; FP64 in both precision modes) */
+int z2_propgr_K_R(z2_ctx *ctx, int32_t nt);  /* propgrK_R  src/multiply.f90:43-63 */
+int z2_propgr_T_R(z2_ctx *ctx, int32_t nt);  /* propgrT_R  src/multiply.f90:110-130 */
+/* LocalK_mod (src/localK.f90) */
+int z2_localK_prop_L(z2_ctx *ctx, int32_t nt); /* LocalK_prop_L src/localK.f90:347-373 */
+int z2_localK_prop_R(z2_ctx *ctx, int32_t nt); /* LocalK_prop_R src/localK.f90:375-398 */
+/* Dynamics_mod / ObserTau_mod / Wrap_tau */
+int z2_dyn_reset(z2_ctx *ctx);                 /* Dyn%reset src/dynamics.f90:41-48; after a left pass */
+int z2_obs_tau_reset(z2_ctx *ctx);             /* Obs_tau%reset src/obser_tau.f90:26-35 */
+int z2_obs_tau_calc(z2_ctx *ctx, int32_t nt);  /* Obs_tau%calc src/obser_tau.f90:56-133 */
+int z2_obs_tau_ave(z2_ctx *ctx, int32_t nobst);/* Obs_tau%ave src/obser_tau.f90:37-47 */
+int z2_wrap_tau(z2_ctx *ctx, int32_t nt);      /* Wrap_tau src/stabilization.f90:478-514 */
 
 /* results / diagnostics */
 int z2_sync(z2_ctx *ctx);
@@ -193,8 +222,15 @@ int z2_get_stream(z2_ctx *ctx, void **stream);
 #define Z2_K_TRSM 7
 #define Z2_K_OBS 8
 #define Z2_K_OBS_TAU 9
+#define Z2_K_QT_APPLY 10  /* Q^T applied as block reflectors in the G recompute */
 int z2_profile_kernel(z2_ctx *ctx, int32_t kind);
 int z2_profile_read(z2_ctx *ctx, double *total_ms, int64_t *launches);
+/* algorithmic work of the launches timed since z2_profile_kernel: flops for
+ * QR (4/3 n^3), explicit Q (4/3 n^3), Q^T application (2 n^3), GEMM (2 m n k),
+ * TRSM (n^3); bytes for the kinetic and observable kernels (read + write of
+ * every matrix, 2 N^2 x element size); 0 for the flip kernel, whose work
+ * (4 N^2 flops per accepted update) the caller takes from z2_get_counters */
+int z2_profile_work(z2_ctx *ctx, double *work);
 
 #ifdef __cplusplus
 }

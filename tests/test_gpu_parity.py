@@ -153,10 +153,13 @@ def test_parity_small_delay_depth_full_sweeps():
 def test_mixed_precision_shadow(L, T, beta, nsw):
     """FP32 propagation replaying the FP64 decisions: max|G32 - G64| before each
     recompute stays small and the observables agree (L = 8, 12, 16: the FP32
-    flip kernels compiled for N = 64, 144, 256)."""
+    flip kernels compiled for N = 64, 144, 256).  Chains 1-3 have mu != 0, so the
+    FP32 paths that apply exp(+-dtau mu) run with real factors: the left pass's
+    factors on read in the flip kernel and in the kinetic prologue, and the right
+    pass's in-kernel final flush (the default delay depth is below 2N)."""
     import z2_gauge_zfy_amd as z
     w = 10
-    specs = [z.ChainSpec(h=0.4 + 0.2 * k, Beta=beta, seed=12345 + 2 * 7919 * k) for k in range(4)]
+    specs = [z.ChainSpec(h=0.4 + 0.2 * k, mu=0.2 * k, Beta=beta, seed=12345 + 2 * 7919 * k) for k in range(4)]
     ref = z.Engine(L, L, T, w, specs, precision=z.Z2_PREC_FP64)
     mix = z.Engine(L, L, T, w, specs, precision=z.Z2_PREC_MIXED)
     for e in (ref, mix):

@@ -103,21 +103,21 @@ def test_tau_requires_enable():
 
 def test_mixed_precision_tau_close_to_fp64():
     """Mixed mode runs the time-displaced path in FP64 from the FP32-propagated G;
-    with the same decisions forced, its correlators stay close to the FP64 run."""
+    after a sweep in which the mixed engine replays the FP64 engine's decisions
+    (z2_sweep_shadow_tau, the measurement at the start of the right pass), its
+    correlators stay close to the FP64 run."""
     import z2_gauge_zfy_amd as z
     L, T, w, beta = 4, 20, 5, 2.0
-    specs = [z.ChainSpec(h=0.5, Beta=beta, seed=4242)]
+    specs = [z.ChainSpec(h=0.5, mu=0.2, Beta=beta, seed=4242)]
     e64 = z.Engine(L, L, T, w, specs)
     e32 = z.Engine(L, L, T, w, specs, precision=z.Z2_PREC_MIXED)
     for e in (e64, e32):
         e.enable_tau()
         e.init_fields()
         e.pre()
-        _gpu_pass_L(e, T, w)
-    if not np.array_equal(e64.sigma(0), e32.sigma(0)):
-        pytest.skip("decisions differ between precisions on this seed")
-    e64.dyn_measure()
-    e32.dyn_measure()
+    e32.sweep_shadow(e64, True, toggle=True)
+    assert np.array_equal(e64.sigma(0), e32.sigma(0)), "the replayed decisions must give the same field"
+    assert e64.seed(0) == e32.seed(0)
     a, b = e64.obs_tau(0), e32.obs_tau(0)
     for k in range(7):
         scale = max(1.0, np.abs(a[k]).max())

@@ -22,7 +22,7 @@ Z2_STAB_DSPLIT = 1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
-              "obs": 8, "obs_tau": 9}
+              "obs": 8, "obs_tau": 9, "qt_apply": 10}
 
 
 class Z2Params(C.Structure):
@@ -63,6 +63,23 @@ _SIGS = {
     "z2_get_dyn_drift": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]),
     "z2_sweep_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_sweep_shadow_tau": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
+                                      C.POINTER(C.c_double)]),
+    "z2_obs_equal_ave": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_sweep_reset": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_prop_K_pre": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_prop_T_pre": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_prop_T_L": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_prop_T_R": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_propgr_K_R": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_propgr_T_R": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_localK_prop_L": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_localK_prop_R": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_dyn_reset": (C.c_int, [C.c_void_p]),
+    "z2_obs_tau_reset": (C.c_int, [C.c_void_p]),
+    "z2_obs_tau_calc": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_obs_tau_ave": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_wrap_tau": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_wrap_pre": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_wrap_L": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "z2_wrap_R": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
@@ -92,6 +109,7 @@ _SIGS = {
     "z2_set_qr_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "z2_profile_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     # kernel test hooks (include/z2dqmc_test.h)
     "z2_test_gemm": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.c_int32,
                                C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_int32,

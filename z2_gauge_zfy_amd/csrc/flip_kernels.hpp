@@ -33,7 +33,8 @@ struct FlipArgs {
   int kdelay;             // delay depth: pending updates before an in-kernel flush
   int *mu_done;           // out: 1 when an in-kernel flush applied the mu factors to G
   int final_flush;        // FP32 only, 1: flush the updates still pending at the end into G and
-                          // write (g*ep)*em (right pass, LocalK_prop_R then propT_R); nacc is then 0
+                          // write (g*ep)*em (right pass, LocalK_prop_R then propT_R); 2: flush
+                          // without the mu factors (LocalK_prop_R alone); nacc is then 0
   const uint32_t *force;  // optional: decisions to replay, bit k = proposal k, chain stride dwords
   uint32_t *record;       // optional: decisions taken, same layout
   int dwords;
@@ -642,7 +643,7 @@ __global__ void __launch_bounds__(512) k_flip(FlipArgs<R> a) {
   }
   if constexpr (sizeof(R) == 4) {
     if (a.final_flush) {
-      flush32(true);
+      flush32(a.final_flush == 1);
       na = 0;
     }
   }

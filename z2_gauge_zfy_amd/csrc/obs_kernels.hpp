@@ -132,12 +132,12 @@ __global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
 }
 
 // per-bin averages (Obs_equal_ave)
-__global__ void k_obs_ave(const double *acc, double *res, const ChainConst *cc, int N, int nch) {
+__global__ void k_obs_ave(const double *acc, double *res, const ChainConst *cc, int N, int nch, int nobs_in) {
   int chain = blockIdx.x * blockDim.x + threadIdx.x;
   if (chain >= nch) return;
   const double *a = acc + (size_t)chain * OBS_NACC;
   double *r = res + (size_t)chain * RES_N;
-  double nobs = a[OBS_NOBS];
+  double nobs = nobs_in > 0 ? (double)nobs_in : a[OBS_NOBS];
   if (nobs <= 0.0) return;
   double z = 1.0 / nobs;
   double fm = a[OBS_FLUX_SUM] * z, fsq = a[OBS_FLUX_SQ] * z;

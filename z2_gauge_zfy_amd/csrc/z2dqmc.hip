@@ -148,10 +148,14 @@ struct z2_ctx {
   char *reorder_tmp = nullptr;
   std::vector<void *> allocs;
   bool have_fields = false, have_pre = false;
+  // 1: every chain has just finished a left pass (the place of Dyn%reset, the start
+  // of a right pass); 0: not; -1: unknown (chains in different directions)
+  int left_done = 0;
   // kernel timing hooks: HIP events around every launch of one kernel class
   int prof_kind = 0;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
   size_t prof_used = 0;
+  double prof_work = 0.0;
 };
 
 static int prof_begin(z2_ctx *c, int kind) {
@@ -166,10 +170,12 @@ static int prof_begin(z2_ctx *c, int kind) {
   return Z2_OK;
 }
 
-static int prof_end(z2_ctx *c, int kind) {
+// work: the launch's algorithmic work (flops, or bytes for the memory-bound classes)
+static int prof_end(z2_ctx *c, int kind, double work) {
   if (c->prof_kind != kind) return Z2_OK;
   HIPC(hipEventRecord(c->prof_ev[c->prof_used].second, c->stream));
   c->prof_used++;
+  c->prof_work += work;
   return Z2_OK;
 }
 
@@ -899,7 +905,7 @@ static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
     case 256: CHK((launch_rows_n<R, 256>(c, a, nmat))); break;
     default: CHK((launch_rows_n<R, 0>(c, a, nmat)));
   }
-  CHK(prof_end(c, Z2_K_KIN_ROWS));
+  CHK(prof_end(c, Z2_K_KIN_ROWS, 2.0 * sizeof(R) * c->N * c->N * (double)nmat * c->nch));
   return Z2_OK;
 }
 
@@ -912,7 +918,7 @@ static int launch_cols(z2_ctx *c, KinArgs<R> &a, int nmat) {
     case 256: CHK((launch_cols_n<R, 256>(c, a, nmat))); break;
     default: CHK((launch_cols_n<R, 0>(c, a, nmat)));
   }
-  CHK(prof_end(c, Z2_K_KIN_COLS));
+  CHK(prof_end(c, Z2_K_KIN_COLS, 2.0 * sizeof(R) * c->N * c->N * (double)nmat * c->nch));
   return Z2_OK;
 }
 
@@ -975,7 +981,22 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read, int fi
   else if (mpt == 2) hipLaunchKernelGGL((k_flip<R, 2>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
   else hipLaunchKernelGGL((k_flip<R, 4>), dim3(c->nch), dim3(FLIP_THREADS), sh, c->stream, f);
   LAUNCH_CHECK();
-  CHK(prof_end(c, Z2_K_FLIP));
+  CHK(prof_end(c, Z2_K_FLIP, 0.0));  // 4 N^2 flops per accepted update: counted by the caller
+  return Z2_OK;
+}
+
+// fill with a value (kernels, not hipMemsetAsync: every step of a sweep stays a
+// kernel or a copy, so that a sweep can be captured in a HIP graph)
+__global__ void __launch_bounds__(256) k_fill(double *x, size_t n, double v) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+
+static int dzero(z2_ctx *c, double *x, size_t n) {
+  if (!n) return Z2_OK;
+  size_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)(nb > 1024 ? 1024 : nb)), dim3(256), 0, c->stream, x, n, 0.0);
+  LAUNCH_CHECK();
   return Z2_OK;
 }
 
@@ -1069,7 +1090,7 @@ static int gemm_g(z2_ctx *c, int M, int Nc, int K, const double *A, int lda, lon
   CHK(prof_begin(c, Z2_K_GEMM));
   hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, c->stream, g);
   LAUNCH_CHECK();
-  CHK(prof_end(c, Z2_K_GEMM));
+  CHK(prof_end(c, Z2_K_GEMM, 2.0 * M * Nc * (double)K * c->nch));
   return Z2_OK;
 }
 
@@ -1266,7 +1287,7 @@ static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n,
     } else {
       CHK(launch_qrb(0, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
     }
-    CHK(prof_end(c, Z2_K_QRP));
+    CHK(prof_end(c, Z2_K_QRP, 4.0 / 3.0 * n * n * (double)n * c->nch));
     return Z2_OK;
   }
   QrpArgs q;
@@ -1279,7 +1300,7 @@ static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n,
   size_t sh = sizeof(double) * (2 * n + 32) + sizeof(int) * 32;
   CHK(prof_begin(c, Z2_K_QRP));
   CHK(launch_qrp(n, dim3(c->nch), sh, c->stream, q));
-  CHK(prof_end(c, Z2_K_QRP));
+  CHK(prof_end(c, Z2_K_QRP, 4.0 / 3.0 * n * n * (double)n * c->nch));
   return Z2_OK;
 }
 
@@ -1304,12 +1325,12 @@ static int orgqr_n(z2_ctx *c, const double *A, const double *tau, double *Qo, in
     } else {
       CHK(launch_qrb(1, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
     }
-    CHK(prof_end(c, Z2_K_ORGQR));
+    CHK(prof_end(c, Z2_K_ORGQR, 4.0 / 3.0 * n * n * (double)n * c->nch));
     return Z2_OK;
   }
   CHK(prof_begin(c, Z2_K_ORGQR));
   CHK(launch_orgqr(n, dim3(c->nch), c->stream, A, tau, Qo));
-  CHK(prof_end(c, Z2_K_ORGQR));
+  CHK(prof_end(c, Z2_K_ORGQR, 4.0 / 3.0 * n * n * (double)n * c->nch));
   return Z2_OK;
 }
 
@@ -1334,12 +1355,13 @@ static int apply_qt(z2_ctx *c, const double *A, const double *B, double *X, int 
     q.xtrans = xtrans;
     q.N = n;
     q.tc = tc;
-    CHK(prof_begin(c, Z2_K_ORGQR));
+    CHK(prof_begin(c, Z2_K_QT_APPLY));
     const int rpl = (n + 63) / 64;
     const size_t sh = qrb_lds_bytes(n, nb, tc);
     if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1));
     else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1));
-    CHK(prof_end(c, Z2_K_ORGQR));
+    // Q^T applied to an n x n right-hand side: sum_j 4 (n - j) n = 2 n^3 flops
+    CHK(prof_end(c, Z2_K_QT_APPLY, 2.0 * n * n * (double)n * c->nch));
     return Z2_OK;
   }
   CHK(orgqr(c, A, c->Q));
@@ -1384,8 +1406,9 @@ static int launch_trsm(int n, int nb, hipStream_t st, size_t lds_max, const doub
 
 static int trsm(z2_ctx *c, const double *R, double *X, int right, int n = 0) {
   CHK(prof_begin(c, Z2_K_TRSM));
-  CHK(launch_trsm(n ? n : c->N, c->nch, c->stream, c->lds_max, R, X, right));
-  CHK(prof_end(c, Z2_K_TRSM));
+  const int m = n ? n : c->N;
+  CHK(launch_trsm(m, c->nch, c->stream, c->lds_max, R, X, right));
+  CHK(prof_end(c, Z2_K_TRSM, (double)m * m * (double)m * c->nch));
   return Z2_OK;
 }
 
@@ -1497,6 +1520,7 @@ static inline size_t slotd(const z2_ctx *c, int st) { return (size_t)st * c->nch
 
 extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect preortho time slice %d", nt);
+  if (!c->have_fields) return set_err(Z2_ESTATE, "fields not initialised (z2_init_fields / z2_set_sigma)");
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
   if (nt != 0) {
@@ -1510,6 +1534,10 @@ extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
     CHK(stab_green(c, nt));
     if (c->mixed) CHK(cvt(c, (const double *)c->Gn, c->Gf));
     else CHK(dcopy(c, c->G, c->Gn, nn));
+    // the end of Local_sweep_pre: G(beta) = G(0) is in place
+    c->isb.assign(c->nch_all, 1);  // the reference's main starts every chain with is_beta = .true.
+    c->have_pre = true;
+    c->left_done = 0;
   }
   if (c->mixed) CHK(cvt(c, (const double *)c->UUR, c->UURf));
   return Z2_OK;
@@ -1532,6 +1560,7 @@ extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
   CHK(dcopy(c, c->VLl + slot(c, st), c->VUL, nn));
   CHK(dcopy(c, c->DLl + slotd(c, st), c->DUL, nd));
   if (c->mixed) CHK(cvt(c, (const double *)c->UUL, c->UULf));
+  if (nt == 0) c->left_done = c->nch == c->nch_all ? 1 : -1;
   return Z2_OK;
 }
 
@@ -1552,6 +1581,7 @@ extern "C" int z2_wrap_R(z2_ctx *c, int32_t nt, int32_t flag) {
   CHK(dcopy(c, c->VRl + slot(c, st), c->VUR, nn));
   CHK(dcopy(c, c->DRl + slotd(c, st), c->DUR, nd));
   if (c->mixed) CHK(cvt(c, (const double *)c->UUR, c->UURf));
+  if (nt == c->T) c->left_done = c->nch == c->nch_all ? 0 : -1;
   return Z2_OK;
 }
 
@@ -1619,7 +1649,7 @@ static int pre_prop(z2_ctx *c, int nt) {
 // ---------------------------------------------------------------- observables
 extern "C" int z2_obs_reset(z2_ctx *c) {
   if (!c) return set_err(Z2_EINVAL, "null ctx");
-  HIPC(hipMemsetAsync(c->obs_acc, 0, sizeof(double) * c->nch * OBS_NACC, c->stream));
+  CHK(dzero(c, c->obs_acc, (size_t)c->nch * OBS_NACC));
   return Z2_OK;
 }
 
@@ -1650,13 +1680,15 @@ extern "C" int z2_obs_equal_calc(z2_ctx *c, int32_t nt) {
   CHK(prof_begin(c, Z2_K_OBS));
   if (c->mixed) CHK(obs_equal_launch<float>(c, c->Gf, nt));
   else CHK(obs_equal_launch<double>(c, c->G, nt));
-  CHK(prof_end(c, Z2_K_OBS));
+  CHK(prof_end(c, Z2_K_OBS, 2.0 * (c->mixed ? 4.0 : 8.0) * c->N * (double)c->N * c->nch));
   return Z2_OK;
 }
 
-static int obs_ave(z2_ctx *c) {
+// Obs_equal%ave (src/obser_equal.f90:50-71); nobs > 0 is the caller's sample
+// count, else the count of z2_obs_equal_calc calls since the reset
+static int obs_ave(z2_ctx *c, int nobs = 0) {
   hipLaunchKernelGGL(k_obs_ave, dim3((c->nch + 63) / 64), dim3(64), 0, c->stream, (const double *)c->obs_acc,
-                     c->obs_res, (const ChainConst *)c->dcc, c->N, c->nch);
+                     c->obs_res, (const ChainConst *)c->dcc, c->N, c->nch, nobs);
   LAUNCH_CHECK();
   return Z2_OK;
 }
@@ -1766,27 +1798,32 @@ static int obs_tau(z2_ctx *c, int nt) {
   CHK(prof_begin(c, Z2_K_OBS_TAU));
   hipLaunchKernelGGL(k_obs_tau, dim3(c->nch), dim3(th), 0, c->stream, o);
   LAUNCH_CHECK();
-  CHK(prof_end(c, Z2_K_OBS_TAU));
+  CHK(prof_end(c, Z2_K_OBS_TAU, 0.0));
+  return Z2_OK;
+}
+
+// Dyn_reset, src/dynamics.f90:41-48: the second right propagator from the current
+// one, G(0,0) = G(t,t) = G(t,0) = G, G(0,t) = -(I - G)
+static int dyn_reset(z2_ctx *c) {
+  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
+  CHK(dcopy(c, c->dUUR, c->UUR, nn));
+  CHK(dcopy(c, c->dVUR, c->VUR, nn));
+  CHK(dcopy(c, c->dDUR, c->DUR, nd));
+  dim3 g = ew_grid(c, (size_t)c->N * c->N);
+  if (c->mixed)
+    hipLaunchKernelGGL((k_dyn_reset<float>), g, dim3(256), 0, c->stream, (const float *)c->Gf, c->G00, c->Gtt,
+                       c->Gt0, c->G0t, c->N);
+  else
+    hipLaunchKernelGGL((k_dyn_reset<double>), g, dim3(256), 0, c->stream, (const double *)c->G, c->G00, c->Gtt,
+                       c->Gt0, c->G0t, c->N);
+  LAUNCH_CHECK();
   return Z2_OK;
 }
 
 // Dyn_reset + Dyn%sweep_R, src/dynamics.f90:41-62: starts from the current G and
 // right propagator, steps G(t,0), G(0,t), G(t,t) and U_R through all slices.
 static int dyn_sweep(z2_ctx *c) {
-  const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
-  CHK(dcopy(c, c->dUUR, c->UUR, nn));
-  CHK(dcopy(c, c->dVUR, c->VUR, nn));
-  CHK(dcopy(c, c->dDUR, c->DUR, nd));
-  {
-    dim3 g = ew_grid(c, (size_t)c->N * c->N);
-    if (c->mixed)
-      hipLaunchKernelGGL((k_dyn_reset<float>), g, dim3(256), 0, c->stream, (const float *)c->Gf, c->G00, c->Gtt,
-                         c->Gt0, c->G0t, c->N);
-    else
-      hipLaunchKernelGGL((k_dyn_reset<double>), g, dim3(256), 0, c->stream, (const double *)c->G, c->G00, c->Gtt,
-                         c->Gt0, c->G0t, c->N);
-    LAUNCH_CHECK();
-  }
+  CHK(dyn_reset(c));
   for (int nt = 1; nt <= c->T; ++nt) {
     CHK(obs_tau(c, nt));
     KinArgs<double> a = kin_args<double>(c, nt);
@@ -1807,8 +1844,200 @@ extern "C" int z2_dyn_measure(z2_ctx *c) {
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_dyn_measure");
-  HIPC(hipMemsetAsync(c->otau, 0, sizeof(double) * c->nch * TAU_NOBS * c->T * c->N, c->stream));
+  if (c->left_done == 0)
+    return set_err(Z2_ESTATE, "the time-displaced measurement starts a right pass: run it after a left pass");
+  CHK(dzero(c, c->otau, (size_t)c->nch * TAU_NOBS * c->T * c->N));
   return dyn_sweep(c);
+}
+
+static int h_zero_sync(z2_ctx *c);
+
+// ---------------------------------------------------------------- fine-grained steps
+// The reference's Multiply_mod / LocalK_mod / ObserEqual / ObserTau / Dynamics /
+// Wrap_tau procedures one by one, for callers that sequence a sweep themselves.
+// Each acts on all chains of the context; composing them in the reference's
+// order (src/local_sweep.f90:106-150) gives the same arithmetic as z2_sweep.
+template <typename R>
+static int mu_elem(z2_ctx *c, int nt, R *m0, int f0a, int f0b, R *m1, int f1) {
+  KinArgs<R> a = kin_args<R>(c, nt);
+  a.m[0] = kmat<R>(m0, 0, 0, f0a, f0b);
+  int nmat = 1;
+  if (m1) {
+    a.m[1] = kmat<R>(m1, 0, 0, f1);
+    nmat = 2;
+  }
+  return launch_elem_kin<R>(c, a, nmat);
+}
+
+static int check_slice(z2_ctx *c, int nt) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (nt < 1 || nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", nt);
+  return Z2_OK;
+}
+
+// propK_pre, src/multiply.f90:8-15: UUR <- B(nt) UUR
+extern "C" int z2_prop_K_pre(z2_ctx *c, int32_t nt) {
+  CHK(check_slice(c, nt));
+  if (c->mixed) {
+    KinArgs<float> a = kin_args<float>(c, nt);
+    a.m[0] = kmat<float>(c->UURf, +1);
+    return launch_rows<float>(c, a, 1);
+  }
+  KinArgs<double> a = kin_args<double>(c, nt);
+  a.m[0] = kmat<double>(c->UUR, +1);
+  return launch_rows<double>(c, a, 1);
+}
+
+// propT_pre, src/multiply.f90:65-77: UUR <- UUR exp(dtau mu)
+extern "C" int z2_prop_T_pre(z2_ctx *c, int32_t nt) {
+  CHK(check_slice(c, nt));
+  if (c->mixed) return mu_elem<float>(c, nt, c->UURf, FAC_EP, FAC_NONE, (float *)nullptr, 0);
+  return mu_elem<double>(c, nt, c->UUR, FAC_EP, FAC_NONE, (double *)nullptr, 0);
+}
+
+// propT_L, src/multiply.f90:79-93: G <- (G e^{+}) e^{-}, UUL <- UUL e^{+}
+extern "C" int z2_prop_T_L(z2_ctx *c, int32_t nt) {
+  CHK(check_slice(c, nt));
+  if (c->mixed) return mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, c->UULf, FAC_EP);
+  return mu_elem<double>(c, nt, c->G, FAC_EP, FAC_EM, c->UUL, FAC_EP);
+}
+
+// propT_R, src/multiply.f90:95-108: G <- (G e^{+}) e^{-}, UUR <- UUR e^{+}
+extern "C" int z2_prop_T_R(z2_ctx *c, int32_t nt) {
+  CHK(check_slice(c, nt));
+  if (c->mixed) return mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, c->UURf, FAC_EP);
+  return mu_elem<double>(c, nt, c->G, FAC_EP, FAC_EM, c->UUR, FAC_EP);
+}
+
+// LocalK_prop_L, src/localK.f90:347-373: flips of bonds 2N..1, then
+// G <- B^{-1} (G B) and UUL <- UUL B (the pending updates flushed first)
+template <typename R>
+static int localK_L_impl(z2_ctx *c, int nt) {
+  CHK(launch_flip<R>(c, nt, 1, 0));
+  KinArgs<R> a = kin_args<R>(c, nt);
+  a.m[0] = kmat<R>(PB<R>::G(c), +1, 1);
+  a.m[1] = kmat<R>(PB<R>::UUL(c), +1);
+  CHK(launch_cols<R>(c, a, 2));
+  a.m[0] = kmat<R>(PB<R>::G(c), -1);
+  return launch_rows<R>(c, a, 1);
+}
+
+// LocalK_prop_R, src/localK.f90:375-398: G <- (B G) B^{-1}, flips of bonds
+// 1..2N, the pending updates flushed, then UUR <- B UUR
+template <typename R>
+static int localK_R_impl(z2_ctx *c, int nt) {
+  KinArgs<R> a = kin_args<R>(c, nt);
+  a.m[0] = kmat<R>(PB<R>::G(c), +1);
+  CHK(launch_rows<R>(c, a, 1));
+  a.m[0] = kmat<R>(PB<R>::G(c), -1);
+  CHK(launch_cols<R>(c, a, 1));
+  const int in_kernel = sizeof(R) == 4 && c->kdelay < c->N2;
+  CHK(launch_flip<R>(c, nt, 0, 0, in_kernel ? 2 : 0));  // 2: final flush without the mu factors
+  if (!in_kernel) {
+    a.m[0] = kmat<R>(PB<R>::G(c), 0, 1);
+    CHK(launch_elem_kin<R>(c, a, 1));
+  }
+  a.m[0] = kmat<R>(PB<R>::UUR(c), +1);
+  return launch_rows<R>(c, a, 1);
+}
+
+extern "C" int z2_localK_prop_L(z2_ctx *c, int32_t nt) {
+  CHK(check_slice(c, nt));
+  return c->mixed ? localK_L_impl<float>(c, nt) : localK_L_impl<double>(c, nt);
+}
+
+extern "C" int z2_localK_prop_R(z2_ctx *c, int32_t nt) {
+  CHK(check_slice(c, nt));
+  return c->mixed ? localK_R_impl<float>(c, nt) : localK_R_impl<double>(c, nt);
+}
+
+static int need_tau(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
+  return Z2_OK;
+}
+
+// propgrK_R, src/multiply.f90:43-63: G(t,0) <- B G(t,0), G(0,t) <- G(0,t) B^{-1},
+// G(t,t) <- B G(t,t) B^{-1}, U_R <- B U_R on the time-displaced state (FP64)
+extern "C" int z2_propgr_K_R(z2_ctx *c, int32_t nt) {
+  CHK(need_tau(c));
+  CHK(check_slice(c, nt));
+  KinArgs<double> a = kin_args<double>(c, nt);
+  a.m[0] = kmat<double>(c->Gt0, +1);
+  a.m[1] = kmat<double>(c->Gtt, +1);
+  a.m[2] = kmat<double>(c->dUUR, +1);
+  CHK(launch_rows<double>(c, a, 3));
+  a.m[0] = kmat<double>(c->G0t, -1);
+  a.m[1] = kmat<double>(c->Gtt, -1);
+  return launch_cols<double>(c, a, 2);
+}
+
+// propgrT_R, src/multiply.f90:110-130: G(t,0) e^{+}, G(0,t) e^{-}, (G(t,t) e^{+}) e^{-}, U_R e^{+}
+extern "C" int z2_propgr_T_R(z2_ctx *c, int32_t nt) {
+  CHK(need_tau(c));
+  CHK(check_slice(c, nt));
+  KinArgs<double> a = kin_args<double>(c, nt);
+  a.m[0] = kmat<double>(c->Gt0, 0, 0, FAC_EP);
+  a.m[1] = kmat<double>(c->G0t, 0, 0, FAC_EM);
+  a.m[2] = kmat<double>(c->Gtt, 0, 0, FAC_EP, FAC_EM);
+  a.m[3] = kmat<double>(c->dUUR, 0, 0, FAC_EP);
+  return launch_elem_kin<double>(c, a, 4);
+}
+
+// Dyn%reset, src/dynamics.f90:41-48 (Propagator%asgn, Propgr_reset src/process_matrix.f90:88-96)
+extern "C" int z2_dyn_reset(z2_ctx *c) {
+  CHK(need_tau(c));
+  if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_dyn_reset");
+  if (c->left_done == 0)
+    return set_err(Z2_ESTATE, "Dyn%%reset belongs at the start of a right pass (after a left pass)");
+  return dyn_reset(c);
+}
+
+// Obs_tau%calc, src/obser_tau.f90:56-133
+extern "C" int z2_obs_tau_calc(z2_ctx *c, int32_t nt) {
+  CHK(need_tau(c));
+  CHK(check_slice(c, nt));
+  return obs_tau(c, nt);
+}
+
+// Obs_tau%reset (src/obser_tau.f90:26-35) and Obs_tau%ave (:37-47)
+extern "C" int z2_obs_tau_reset(z2_ctx *c) {
+  CHK(need_tau(c));
+  return dzero(c, c->otau, (size_t)c->nch * TAU_NOBS * c->T * c->N);
+}
+
+extern "C" int z2_obs_tau_ave(z2_ctx *c, int32_t nobst) {
+  CHK(need_tau(c));
+  if (nobst < 1) return set_err(Z2_EINVAL, "Nobst must be positive");
+  hipLaunchKernelGGL(k_scale, dim3(256), dim3(256), 0, c->stream, c->otau,
+                     (size_t)c->nch * TAU_NOBS * c->T * c->N, 1.0 / (double)nobst);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+// Wrap_tau, src/stabilization.f90:478-514
+extern "C" int z2_wrap_tau(z2_ctx *c, int32_t nt) {
+  CHK(need_tau(c));
+  if (nt < 1 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect orthobig time slice %d", nt);
+  return wrap_tau(c, nt);
+}
+
+// Obs_equal%ave(Nobs), src/obser_equal.f90:50-71
+extern "C" int z2_obs_equal_ave(z2_ctx *c, int32_t nobs) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (nobs < 1) return set_err(Z2_EINVAL, "Nobs must be positive");
+  return obs_ave(c, nobs);
+}
+
+// Local_sweep_reset, src/local_sweep.f90:55-71: observables (and with toggle the
+// time-displaced accumulators) zeroed, the field synchronised along tau where h ~ 0
+extern "C" int z2_sweep_reset(z2_ctx *c, int32_t toggle) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (toggle && !c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
+  CHK(h_zero_sync(c));
+  CHK(z2_obs_reset(c));
+  if (toggle) CHK(dzero(c, c->otau, (size_t)c->nch * TAU_NOBS * c->T * c->N));
+  return Z2_OK;
 }
 
 extern "C" int z2_get_obs_tau(z2_ctx *c, int32_t chain, double *out) {
@@ -1861,8 +2090,6 @@ extern "C" int z2_pre(z2_ctx *c) {
     CHK(pre_prop(c, nt));
     if (nt % c->w == 0) CHK(z2_wrap_pre(c, nt));
   }
-  c->isb.assign(c->nch_all, 1);  // the reference's main starts every chain with is_beta = .true.
-  c->have_pre = true;
   return Z2_OK;
 }
 
@@ -2017,7 +2244,7 @@ extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
   CHK(h_zero_sync(c));
   CHK(z2_obs_reset(c));
   const size_t ntau = (size_t)c->nch * TAU_NOBS * c->T * c->N;
-  if (toggle) HIPC(hipMemsetAsync(c->otau, 0, sizeof(double) * ntau, c->stream));
+  if (toggle) CHK(dzero(c, c->otau, ntau));
   int nobst = 0;
   if (is_beta != Z2_BETA_PER_CHAIN) {
     c->isb.assign(c->nch_all, is_beta);
@@ -2157,6 +2384,7 @@ extern "C" int z2_global_sweep(z2_ctx *c, int32_t nglobal) {
     for (int j = 0; j < c->nch_all; ++j)
       if (acc[j]) c->isb[j] = !c->isb[j];
   }
+  c->left_done = -1;  // trial passes of either direction, some restored
   hipLaunchKernelGGL(k_wolff_finish, g1, b1, 0, c->stream, glob_args(c), (int)nglobal);
   LAUNCH_CHECK();
   return Z2_OK;
@@ -2212,7 +2440,7 @@ static int maxdiff(z2_ctx *ref, z2_ctx *c, const Rb *Gc) {
   return Z2_OK;
 }
 
-extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *maxdiff_out) {
+extern "C" int z2_sweep_shadow_tau(z2_ctx *c, z2_ctx *ref, int32_t is_beta, int32_t toggle, double *maxdiff_out) {
   if (!c || !ref || !maxdiff_out) return set_err(Z2_EINVAL, "null argument");
   if (ref->mixed || c->N != ref->N || c->T != ref->T || c->w != ref->w || c->nch != ref->nch ||
       c->p.Nsweep != ref->p.Nsweep)
@@ -2220,21 +2448,21 @@ extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *
   if (!c->have_pre || !ref->have_pre) return set_err(Z2_ESTATE, "z2_pre must run on both contexts");
   if (c->phys != ref->phys) return set_err(Z2_ESTATE, "shadow needs both contexts' chains in the same storage order");
   if (is_beta != 0 && is_beta != 1) return set_err(Z2_EINVAL, "shadow sweeps take is_beta 0 or 1");
-  CHK(h_zero_sync(c));
-  CHK(h_zero_sync(ref));
-  CHK(z2_obs_reset(c));
-  CHK(z2_obs_reset(ref));
-  HIPC(hipMemsetAsync(c->cmp, 0, sizeof(double) * c->nch, c->stream));
+  if (toggle && (!c->tau_on || !ref->tau_on))
+    return set_err(Z2_ESTATE, "time-displaced measurement not enabled on both contexts (z2_enable_tau)");
+  for (z2_ctx *x : {c, ref}) CHK(z2_sweep_reset(x, toggle));
+  CHK(dzero(c, c->cmp, (size_t)c->nch));
   std::vector<Step> v;
-  sweep_steps(ref, is_beta, 0, v);
+  sweep_steps(ref, is_beta, toggle, v);
   c->force = ref->rec;
-  int rc = Z2_OK;
+  int rc = Z2_OK, nobst = 0;
   for (const Step &st : v) {
     if (step_recomputes(ref, st)) {  // propagated G of both, before the recompute
       HIPC(hipStreamSynchronize(ref->stream));
       rc = c->mixed ? maxdiff<float>(ref, c, c->Gf) : maxdiff<double>(ref, c, c->G);
       if (rc) break;
     }
+    nobst += st.kind == ST_DYN;
     if ((rc = run_step(ref, st))) break;
     HIPC(hipStreamSynchronize(ref->stream));
     if ((rc = run_step(c, st))) break;
@@ -2242,11 +2470,20 @@ extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *
   }
   c->force = nullptr;
   CHK(rc);
-  CHK(obs_ave(c));
-  CHK(obs_ave(ref));
-  HIPC(hipMemcpyAsync(maxdiff_out, c->cmp, sizeof(double) * c->nch, hipMemcpyDeviceToHost, c->stream));
+  for (z2_ctx *x : {c, ref}) {
+    CHK(obs_ave(x));
+    if (nobst > 0) CHK(z2_obs_tau_ave(x, nobst));
+  }
+  // cmp is indexed by storage slot; the caller's index is the chain's own
+  std::vector<double> slotv(c->nch_all);
+  HIPC(hipMemcpyAsync(slotv.data(), c->cmp, sizeof(double) * c->nch_all, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->nch_all; ++k) maxdiff_out[k] = slotv[c->phys[k]];
   return Z2_OK;
+}
+
+extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *maxdiff_out) {
+  return z2_sweep_shadow_tau(c, ref, is_beta, 0, maxdiff_out);
 }
 
 // ---------------------------------------------------------------- kernel test hooks
@@ -2382,10 +2619,11 @@ extern "C" int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right
 
 // ---------------------------------------------------------------- kernel timing hooks
 extern "C" int z2_profile_kernel(z2_ctx *c, int32_t kind) {
-  if (!c || kind < 0 || kind > Z2_K_OBS_TAU) return set_err(Z2_EINVAL, "bad kernel class %d", kind);
+  if (!c || kind < 0 || kind > Z2_K_QT_APPLY) return set_err(Z2_EINVAL, "bad kernel class %d", kind);
   HIPC(hipStreamSynchronize(c->stream));
   c->prof_kind = kind;
   c->prof_used = 0;
+  c->prof_work = 0.0;
   return Z2_OK;
 }
 
@@ -2400,5 +2638,11 @@ extern "C" int z2_profile_read(z2_ctx *c, double *total_ms, int64_t *launches) {
   }
   *total_ms = t;
   *launches = (int64_t)c->prof_used;
+  return Z2_OK;
+}
+
+extern "C" int z2_profile_work(z2_ctx *c, double *work) {
+  if (!c || !work) return set_err(Z2_EINVAL, "null argument");
+  *work = c->prof_work;
   return Z2_OK;
 }

@@ -133,12 +133,65 @@ class Engine:
         check(lib().z2_get_dyn_drift(self._ptr, chain, _pd(a), _pd(b)))
         return a, b
 
-    def sweep_shadow(self, ref: "Engine", is_beta: bool = True) -> np.ndarray:
+    def sweep_shadow(self, ref: "Engine", is_beta: bool = True, toggle: bool = False) -> np.ndarray:
         """One sweep in lockstep with an FP64 reference engine, replaying its
-        decisions; returns max|G_self - G_ref| per chain before each recompute."""
+        decisions (toggle: with the time-displaced measurement on both); returns
+        max|G_self - G_ref| per chain before each recompute."""
         out = np.zeros(self.nch)
-        check(lib().z2_sweep_shadow(self._ptr, ref.ptr, int(is_beta), _pd(out)))
+        check(lib().z2_sweep_shadow_tau(self._ptr, ref.ptr, int(is_beta), int(bool(toggle)), _pd(out)))
         return out
+
+    # ---- fine-grained steps (reference procedure granularity), all chains at slice nt
+    def _step(self, name, *args):
+        check(getattr(lib(), name)(self._ptr, *[int(x) for x in args]))
+
+    def sweep_reset(self, toggle=False):
+        self._step("z2_sweep_reset", bool(toggle))
+
+    def prop_K_pre(self, nt):
+        self._step("z2_prop_K_pre", nt)
+
+    def prop_T_pre(self, nt):
+        self._step("z2_prop_T_pre", nt)
+
+    def prop_T_L(self, nt):
+        self._step("z2_prop_T_L", nt)
+
+    def prop_T_R(self, nt):
+        self._step("z2_prop_T_R", nt)
+
+    def propgr_K_R(self, nt):
+        self._step("z2_propgr_K_R", nt)
+
+    def propgr_T_R(self, nt):
+        self._step("z2_propgr_T_R", nt)
+
+    def localK_prop_L(self, nt):
+        self._step("z2_localK_prop_L", nt)
+
+    def localK_prop_R(self, nt):
+        self._step("z2_localK_prop_R", nt)
+
+    def obs_equal_calc(self, nt):
+        self._step("z2_obs_equal_calc", nt)
+
+    def obs_equal_ave(self, nobs):
+        self._step("z2_obs_equal_ave", nobs)
+
+    def dyn_reset(self):
+        self._step("z2_dyn_reset")
+
+    def obs_tau_reset(self):
+        self._step("z2_obs_tau_reset")
+
+    def obs_tau_calc(self, nt):
+        self._step("z2_obs_tau_calc", nt)
+
+    def obs_tau_ave(self, nobst):
+        self._step("z2_obs_tau_ave", nobst)
+
+    def wrap_tau(self, nt):
+        self._step("z2_wrap_tau", nt)
 
     def wrap_pre(self, nt):
         check(lib().z2_wrap_pre(self._ptr, nt))
@@ -212,6 +265,12 @@ class Engine:
         t, n = C.c_double(), C.c_int64()
         check(lib().z2_profile_read(self._ptr, C.byref(t), C.byref(n)))
         return t.value, n.value
+
+    def profile_work(self) -> float:
+        """algorithmic flops (or bytes) of the launches timed since profile_kernel"""
+        w = C.c_double()
+        check(lib().z2_profile_work(self._ptr, C.byref(w)))
+        return w.value
 
     def stream(self) -> int:
         s = C.c_void_p()
