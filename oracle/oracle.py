@@ -113,6 +113,7 @@ def lib():
             "z2o_get_acc": (None, [V, PD]),
             "z2o_get_obs_equal": (None, [V, C.POINTER(ObsEqual)]),
             "z2o_get_obs_tau": (None, [V, PD]),
+            "z2o_obs_equal_terms": (None, [V, C.c_int, PD]),
             "z2o_get_dyn_drift": (None, [V, PD, PD]),
             "z2o_get_min_gap": (D, [V]),
             "z2o_reset_min_gap": (None, [V]),
@@ -367,6 +368,13 @@ class Chain:
         o = ObsEqual()
         lib().z2o_get_obs_equal(self.ptr, C.byref(o))
         return o.as_dict()
+
+    def obs_equal_terms(self, nt):
+        """the sums Obs_equal_calc adds for slice nt on the current G:
+        flux_tau, pair_sum, cdw_sum, cdw_shift_sum, density_slice"""
+        o = np.zeros(5)
+        lib().z2o_obs_equal_terms(self.ptr, nt, _pd(o))
+        return dict(zip(("flux_tau", "pair_sum", "cdw_sum", "cdw_shift_sum", "density_slice"), o))
 
     def obs_tau(self):
         o = np.zeros(7 * self.N * self.T)

@@ -1620,12 +1620,12 @@ void z2o_slice_R(z2o_chain *c, int nt) {
  * ====================================================================== */
 static void obs_equal_reset(obseq_t *o) { memset(o, 0, sizeof(*o)); }
 
-/* Obs_equal_calc, src/obser_equal.f90:73-149 */
-static void obs_equal_calc(z2o_chain *c, int nt) {
+/* the per-slice sums of Obs_equal_calc, src/obser_equal.f90:73-149:
+ * out = {flux_tau, pair_sum, cdw_sum, cdw_shift_sum, density_slice} */
+static void obs_equal_terms(const z2o_chain *c, int nt, double out[5]) {
   const z2o_model *m = c->m;
   const int N = c->N, N2 = c->N2;
   const double *G = c->prop.Gr;
-  obseq_t *o = &c->oe;
   const double PI = acos(-1.0);
   double qshift_x = PI - 2.0 * PI / (double)m->p.Nlx;
   double flux_tau = 0.0;
@@ -1642,8 +1642,6 @@ static void obs_equal_calc(z2o_chain *c, int nt) {
                               SIG(c->sigma, bt, nt, N2) * SIG(c->sigma, bl, nt, N2);
   }
   flux_tau = flux_tau / (double)N;
-  o->flux_sum = o->flux_sum + flux_tau;
-  o->flux_sq = o->flux_sq + flux_tau * flux_tau;
   double pair_sum = 0.0, cdw_sum = 0.0, cdw_shift_sum = 0.0, density_slice = 0.0;
   for (int ii = 0; ii < N; ++ii) {
     double n_i = 1.0 - G[ii + (size_t)ii * N];
@@ -1667,10 +1665,26 @@ static void obs_equal_calc(z2o_chain *c, int nt) {
       cdw_shift_sum = cdw_shift_sum + phase_shift * dc;
     }
   }
-  o->pair_sc_acc = o->pair_sc_acc + pair_sum;
-  o->cdw_pi_acc = o->cdw_pi_acc + cdw_sum;
-  o->cdw_shift_acc = o->cdw_shift_acc + cdw_shift_sum;
-  o->density = o->density + density_slice / (double)N;
+  out[0] = flux_tau;
+  out[1] = pair_sum;
+  out[2] = cdw_sum;
+  out[3] = cdw_shift_sum;
+  out[4] = density_slice;
+}
+
+void z2o_obs_equal_terms(const z2o_chain *c, int nt, double out[5]) { obs_equal_terms(c, nt, out); }
+
+/* Obs_equal_calc, src/obser_equal.f90:73-149 */
+static void obs_equal_calc(z2o_chain *c, int nt) {
+  obseq_t *o = &c->oe;
+  double t[5];
+  obs_equal_terms(c, nt, t);
+  o->flux_sum = o->flux_sum + t[0];
+  o->flux_sq = o->flux_sq + t[0] * t[0];
+  o->pair_sc_acc = o->pair_sc_acc + t[1];
+  o->cdw_pi_acc = o->cdw_pi_acc + t[2];
+  o->cdw_shift_acc = o->cdw_shift_acc + t[3];
+  o->density = o->density + t[4] / (double)c->N;
 }
 
 /* Obs_equal_ave, src/obser_equal.f90:50-71 */

@@ -127,6 +127,10 @@ void z2o_get_counters(const z2o_chain *c, double *out);
  * [3] Wolff%size_cluster */
 void z2o_get_acc(const z2o_chain *c, double *out);
 void z2o_get_obs_equal(const z2o_chain *c, z2o_obs_equal *out);
+/* the per-slice sums Obs_equal_calc adds for slice nt on the current G
+ * (src/obser_equal.f90:73-149): flux_tau, pair_sum, cdw_sum, cdw_shift_sum,
+ * density_slice; the accumulators are not touched */
+void z2o_obs_equal_terms(const z2o_chain *c, int nt, double out[5]);
 /* time-displaced arrays (real parts), 7 x N x Ltrot in the order
  * spin, pair_s, pair_d, charge_s, charge_d, single, curxx, col-major (imj,nt) */
 void z2o_get_obs_tau(const z2o_chain *c, double *out);
