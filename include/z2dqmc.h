@@ -206,6 +206,19 @@ int z2_set_stab_mode(z2_ctx *ctx, int32_t mode);
 #define Z2_QR_BLOCKED 0
 #define Z2_QR_PIVOTED 1
 int z2_set_qr_mode(z2_ctx *ctx, int32_t mode);
+/* QDRP_decompose (src/stabilization.f90:42-68) for nmat n x n matrices that the
+ * caller holds in device memory (column-major, matrix stride n*n; n <= 512),
+ * enqueued on `stream` (hipStream_t as void*, NULL: the null stream):
+ * A P = Q D R with the columns ordered by decreasing norm and a blocked
+ * Householder QR.  On return A holds R with its rows scaled to unit diagonal
+ * magnitude in the upper triangle and the Householder vectors below, D = |diag R|
+ * (stride n), ipvt the 1-based column order (stride n), tau the reflector
+ * scalars (stride n), and Q (optional, stride n*n) the explicit orthogonal
+ * factor.  The library allocates a workspace of nmat n^2 doubles for the call
+ * and releases it before returning (the call waits for the stream). */
+int z2_qdrp_batched(int32_t n, int32_t nmat, double *A, double *D, int32_t *ipvt, double *tau, double *Q,
+                    void *stream);
+
 /* HIP stream of the context (hipStream_t as void*) for timing by the caller */
 int z2_get_stream(z2_ctx *ctx, void **stream);
 

@@ -95,3 +95,62 @@ def test_trsm(L, n, right):
     check(L.z2_test_trsm(n, _pd(R), _pd(X), right))
     want = np.linalg.solve(np.triu(R), X0) if right == 0 else np.linalg.solve(np.triu(R), X0.T).T
     np.testing.assert_allclose(X, want, rtol=1e-10, atol=1e-10)
+
+
+def _hip():
+    """the HIP runtime the engine library itself links (one runtime per process)"""
+    from z2_gauge_zfy_amd import _lib
+    _lib.lib()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return h
+
+
+def qdrp_on_device(fn, mats):
+    """run a batched QDRP entry point on device copies of `mats` ([nmat, n, n])
+    through the C ABI (null stream); returns A, D, ipvt, tau, Q on the host"""
+    from z2_gauge_zfy_amd._lib import check
+    h = _hip()
+    nmat, n, _ = mats.shape
+    host = {"A": np.ascontiguousarray(np.transpose(mats, (0, 2, 1))),  # column-major per matrix
+            "D": np.zeros((nmat, n)), "ip": np.zeros((nmat, n), np.int32), "tau": np.zeros((nmat, n)),
+            "Q": np.zeros((nmat, n, n))}
+    dev = {}
+    try:
+        for k, a in host.items():
+            p = C.c_void_p()
+            assert h.hipMalloc(C.byref(p), a.nbytes) == 0
+            dev[k] = p
+            assert h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        check(fn(n, nmat, dev["A"], dev["D"], dev["ip"], dev["tau"], dev["Q"], None))
+        assert h.hipDeviceSynchronize() == 0
+        for k, a in host.items():
+            assert h.hipMemcpy(a.ctypes.data, dev[k], a.nbytes, 2) == 0  # device to host
+    finally:
+        for p in dev.values():
+            h.hipFree(p)
+    t = lambda x: np.transpose(x, (0, 2, 1))  # back to [nmat][row][col]
+    return t(host["A"]), host["D"], host["ip"], host["tau"], t(host["Q"])
+
+
+@pytest.mark.parametrize("n, nmat", [(144, 3), (256, 2)])
+def test_qdrp_batched_device_pointers(L, n, nmat):
+    """z2_qdrp_batched on matrices in device memory: each matrix's factorization is
+    the single-matrix blocked QR's, bit for bit, and A P = Q diag(D) R"""
+    rng = np.random.default_rng(7 + n)
+    mats = np.stack([rng.standard_normal((n, n)) @ np.diag(np.logspace(0, 8, n)) for _ in range(nmat)])
+    A, D, ip, tau, Q = qdrp_on_device(L.z2_qdrp_batched, mats)
+    for k in range(nmat):
+        A1 = np.asfortranarray(mats[k].copy())
+        D1, ip1, tau1 = np.zeros(n), np.zeros(n, np.int32), np.zeros(n)
+        Q1 = np.zeros((n, n), order="F")
+        from z2_gauge_zfy_amd._lib import check
+        check(L.z2_test_qr_blocked(n, _pd(A1), _pd(D1), ip1.ctypes.data_as(C.POINTER(C.c_int32)), _pd(tau1),
+                                   _pd(Q1)))
+        assert np.array_equal(A[k], A1) and np.array_equal(D[k], D1)
+        assert np.array_equal(ip[k], ip1) and np.array_equal(tau[k], tau1) and np.array_equal(Q[k], Q1)
+        R = np.triu(A[k]) * D[k][:, None]
+        np.testing.assert_allclose(Q[k] @ R, mats[k][:, ip[k] - 1], rtol=1e-11,
+                                   atol=1e-11 * np.abs(mats[k]).max())

@@ -2486,6 +2486,50 @@ extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *
   return z2_sweep_shadow_tau(c, ref, is_beta, 0, maxdiff_out);
 }
 
+// ---------------------------------------------------------------- batched QDRP
+// QDRP_decompose (src/stabilization.f90:42-68) on caller matrices in device
+// memory: A P = Q D R by the blocked Householder QR with the columns ordered by
+// decreasing norm, one workgroup per matrix.  W is the QR's staging copy of the
+// partially reduced columns (nmat n x n doubles); every element of W is written
+// before it is read, so it needs no clearing.
+static int qdrp_batched(int n, int nmat, double *A, double *D, int32_t *ipvt, double *tau, double *Q,
+                        hipStream_t st, double *W) {
+  int dev = 0, lds = 65536;
+  HIPC(hipGetDevice(&dev));
+  HIPC(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  int nb = 0, tc = 0;
+  if (!qrb_config((size_t)lds, n, &nb, &tc)) return set_err(Z2_EINVAL, "order %d too large for the blocked QR", n);
+  QrbArgs q;
+  memset(&q, 0, sizeof(q));
+  q.A = A; q.W = W; q.tau = tau; q.ipvt = ipvt; q.D = D; q.N = n; q.tc = tc; q.nopiv = 0;
+  q.Ain = A; q.tin = tau; q.Q = Q;
+  const size_t sh = qrb_lds_bytes(n, nb, tc);
+  CHK(launch_qrb(0, nb, nmat, sh, st, q));
+  if (Q) CHK(launch_qrb(1, nb, nmat, sh, st, q));
+  return Z2_OK;
+}
+
+static int qdrp_args(int32_t n, int32_t nmat, const double *A, const double *D, const int32_t *ipvt,
+                     const double *tau) {
+  if (n < 1 || nmat < 1 || !A || !D || !ipvt || !tau) return set_err(Z2_EINVAL, "bad argument");
+  return Z2_OK;
+}
+
+extern "C" int z2_qdrp_batched(int32_t n, int32_t nmat, double *A, double *D, int32_t *ipvt, double *tau,
+                               double *Q, void *stream) {
+  CHK(qdrp_args(n, nmat, A, D, ipvt, tau));
+  hipStream_t st = (hipStream_t)stream;
+  void *W = nullptr;
+  hipError_t e = hipMalloc(&W, sizeof(double) * (size_t)nmat * n * n);
+  if (e != hipSuccess) return set_err(Z2_ENOMEM, "hipMalloc(workspace): %s", hipGetErrorString(e));
+  int rc = qdrp_batched(n, nmat, A, D, ipvt, tau, Q, st, (double *)W);
+  hipError_t es = hipStreamSynchronize(st);  // W is released on return
+  (void)hipFree(W);
+  if (rc) return rc;
+  if (es != hipSuccess) return set_err(Z2_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(es));
+  return Z2_OK;
+}
+
 // ---------------------------------------------------------------- kernel test hooks
 // (include/z2dqmc_test.h) single-matrix runs of the dense kernels for parity tests.
 namespace {
