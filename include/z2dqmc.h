@@ -219,6 +219,18 @@ int z2_set_qr_mode(z2_ctx *ctx, int32_t mode);
 int z2_qdrp_batched(int32_t n, int32_t nmat, double *A, double *D, int32_t *ipvt, double *tau, double *Q,
                     void *stream);
 
+/* No counterpart in the reference: z2_qdrp_batched for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results; the workspace
+ * is a static device array of Z2_QDRP_SCRATCH_DOUBLES doubles (32 MiB) that
+ * each GPU holds until the process ends, so nmat * n^2 must not exceed it
+ * (Z2_EINVAL otherwise).  Calls nothing that allocates and does not wait for
+ * the stream.  Every call on one GPU uses the same array: calls that overlap on
+ * one GPU race. */
+#define Z2_QDRP_SCRATCH_DOUBLES 4194304
+int z2_qdrp_batched_noalloc(int32_t n, int32_t nmat, double *A, double *D, int32_t *ipvt, double *tau, double *Q,
+                            void *stream);
+
 /* HIP stream of the context (hipStream_t as void*) for timing by the caller */
 int z2_get_stream(z2_ctx *ctx, void **stream);
 

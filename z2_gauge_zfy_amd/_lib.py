@@ -112,6 +112,8 @@ _SIGS = {
     "z2_profile_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "z2_qdrp_batched": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+    "z2_qdrp_batched_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]),
     # kernel test hooks (include/z2dqmc_test.h)
     "z2_test_gemm": (C.c_int, [C.c_int32, C.POINTER(C.c_double), C.c_int32,
                                C.POINTER(C.c_double), C.c_int32, C.c_int32, C.c_int32,

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2528,6 +2529,46 @@ extern "C" int z2_qdrp_batched(int32_t n, int32_t nmat, double *A, double *D, in
   if (rc) return rc;
   if (es != hipSuccess) return set_err(Z2_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(es));
   return Z2_OK;
+}
+
+// The same without allocation: the QR's staging copy W lives in `scratch`, a
+// file-scope device array (every GPU the library runs on holds its own copy
+// until the process ends).  In host code `scratch` names a host-side stand-in,
+// not a device address: its device address is taken once per device with
+// hipGetSymbolAddress and only that address is handed to kernels.  All calls on
+// one GPU share the array, so calls that overlap on one GPU (two host threads, or
+// two streams without an ordering between them) race on it.
+static __device__ __align__(256) double scratch[Z2_QDRP_SCRATCH_DOUBLES];
+
+static constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *g_scratch_dev[SCRATCH_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of this device's `scratch` (the device must be current)
+static int scratch_ptr(double **out) {
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return set_err(Z2_EINVAL, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return set_err(Z2_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_dev[dev] = (double *)p;
+  }
+  *out = g_scratch_dev[dev];
+  return Z2_OK;
+}
+
+extern "C" int z2_qdrp_batched_noalloc(int32_t n, int32_t nmat, double *A, double *D, int32_t *ipvt, double *tau,
+                                       double *Q, void *stream) {
+  CHK(qdrp_args(n, nmat, A, D, ipvt, tau));
+  if ((size_t)nmat * n * n > (size_t)Z2_QDRP_SCRATCH_DOUBLES)
+    return set_err(Z2_EINVAL, "nmat * n^2 = %zu exceeds the %d doubles of the static workspace", (size_t)nmat * n * n,
+                   (int)Z2_QDRP_SCRATCH_DOUBLES);
+  double *W = nullptr;
+  CHK(scratch_ptr(&W));
+  return qdrp_batched(n, nmat, A, D, ipvt, tau, Q, (hipStream_t)stream, W);
 }
 
 // ---------------------------------------------------------------- kernel test hooks
