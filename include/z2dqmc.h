@@ -195,6 +195,10 @@ int z2_set_delay(z2_ctx *ctx, int32_t k);
  * large beta where the reference's form loses it; it is a deviation from the
  * reference's arithmetic (G agrees to rounding where both are accurate), for
  * opt-in use.  The time-displaced recompute is not affected. */
+/* kinetic tiles (no counterpart in the reference): columns per workgroup of the
+ * row-rotation kernel (8 or 16) and rows per workgroup of the column-rotation
+ * kernel (8 or 32); results do not depend on them (default 8, 8) */
+int z2_set_kin_tiles(z2_ctx *ctx, int32_t rows_cb, int32_t cols_rb);
 #define Z2_STAB_REFERENCE 0
 #define Z2_STAB_DSPLIT 1
 int z2_set_stab_mode(z2_ctx *ctx, int32_t mode);
@@ -233,6 +237,52 @@ int z2_qdrp_batched_noalloc(int32_t n, int32_t nmat, double *A, double *D, int32
 
 /* HIP stream of the context (hipStream_t as void*) for timing by the caller */
 int z2_get_stream(z2_ctx *ctx, void **stream);
+
+/* ---- batches of chains of uneven size (no counterpart in the reference, which
+ * runs one chain per MPI rank, src/main.f90:23-25).  Every chain has its own
+ * lattice, Ltrot, Nwrap, couplings and seed.  Chains of one shape (Nlx, Nly,
+ * Ltrot, Nwrap) form a group -- one context, on its own HIP stream, split into
+ * near-equal parts of at most max_group chains when max_group > 0 -- and a sweep
+ * of the batch advances every group: Z2_EXEC_EAGER issues the groups' launches in
+ * turn on their streams, Z2_EXEC_GRAPH replays one captured HIP graph holding
+ * all groups' launches as parallel branches (one graph launch per sweep; the
+ * first sweep of each kind runs eagerly).  Both give the same results bit for
+ * bit, and the same as one context per group.  The per-chain state and results
+ * are read through the group context: z2_batch_chain_ctx maps a batch chain to
+ * (context, index in it). */
+typedef struct {
+  int32_t Nlx, Nly, Ltrot, Nwrap;
+  z2_chain_desc d;
+} z2_batch_chain;
+typedef struct z2_batch z2_batch;
+#define Z2_EXEC_EAGER 0
+#define Z2_EXEC_GRAPH 1
+int z2_batch_create(const z2_batch_chain *chains, int32_t n_chains, int32_t precision, int32_t Nsweep,
+                    int32_t max_group, z2_batch **out);
+int z2_batch_destroy(z2_batch *b);
+int z2_batch_info(z2_batch *b, int32_t *n_chains, int32_t *n_groups);
+int z2_batch_chain_ctx(z2_batch *b, int32_t chain, z2_ctx **ctx, int32_t *index);
+int z2_batch_group_ctx(z2_batch *b, int32_t group, z2_ctx **ctx);
+int z2_batch_init_fields(z2_batch *b);
+int z2_batch_enable_tau(z2_batch *b);
+int z2_batch_pre(z2_batch *b);
+/* LocalSweep%sweep for every chain: is_beta 0 / 1 for all, or Z2_BETA_PER_CHAIN
+ * (each chain's own direction, eager); toggle: time-displaced measurement */
+int z2_batch_sweep(z2_batch *b, int32_t is_beta, int32_t toggle);
+int z2_batch_sync(z2_batch *b);
+int z2_batch_set_exec(z2_batch *b, int32_t mode);
+int z2_batch_get_exec(z2_batch *b, int32_t *mode);
+/* Autotuner (before z2_batch_pre, after the fields are set): per group, on clones
+ * of its chains after `warm` sweeps, the delay depth k in {4, 8, 16, 32} (up to
+ * the LDS limit) and the kinetic tiles by timing `sweeps` sweeps each, then
+ * Nwrap in {5, 10, 20, own} (dividing Ltrot) as the fastest whose drift
+ * max|G_recomputed - G_propagated| stays <= drift_tol; then the execution mode on
+ * a clone of the whole batch.  The batch's own chains are not advanced: a group
+ * whose Nwrap changes is rebuilt from its chains' fields and seeds.  path (may be
+ * NULL) receives the choices; z2_batch_load_tuning applies such a file (before
+ * z2_batch_pre) to the groups it matches (lattice, Ltrot, precision, chain count). */
+int z2_batch_autotune(z2_batch *b, int32_t warm, int32_t sweeps, double drift_tol, const char *path);
+int z2_batch_load_tuning(z2_batch *b, const char *path);
 
 /* Kernel timing (no counterpart in the reference; for roofline measurement):
  * bracket every launch of one kernel class with HIP events on the context's

@@ -19,6 +19,8 @@ Z2_QR_PIVOTED = 1
 Z2_BETA_PER_CHAIN = -1
 Z2_STAB_REFERENCE = 0
 Z2_STAB_DSPLIT = 1
+Z2_EXEC_EAGER = 0
+Z2_EXEC_GRAPH = 1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
@@ -34,6 +36,11 @@ class Z2ChainDesc(C.Structure):
     _fields_ = [("RT", C.c_double), ("J", C.c_double), ("h", C.c_double), ("mu", C.c_double),
                 ("Q", C.c_double), ("Beta", C.c_double), ("seed", C.c_int32),
                 ("absolute", C.c_int32)]
+
+
+class Z2BatchChain(C.Structure):
+    _fields_ = [("Nlx", C.c_int32), ("Nly", C.c_int32), ("Ltrot", C.c_int32), ("Nwrap", C.c_int32),
+                ("d", Z2ChainDesc)]
 
 
 class Z2Error(RuntimeError):
@@ -110,6 +117,23 @@ _SIGS = {
     "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "z2_profile_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "z2_set_kin_tiles": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                  C.POINTER(C.c_void_p)]),
+    "z2_batch_destroy": (C.c_int, [C.c_void_p]),
+    "z2_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "z2_batch_chain_ctx": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p),
+                                     C.POINTER(C.c_int32)]),
+    "z2_batch_group_ctx": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]),
+    "z2_batch_init_fields": (C.c_int, [C.c_void_p]),
+    "z2_batch_enable_tau": (C.c_int, [C.c_void_p]),
+    "z2_batch_pre": (C.c_int, [C.c_void_p]),
+    "z2_batch_sweep": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_batch_sync": (C.c_int, [C.c_void_p]),
+    "z2_batch_set_exec": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_batch_get_exec": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "z2_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_char_p]),
+    "z2_batch_load_tuning": (C.c_int, [C.c_void_p, C.c_char_p]),
     "z2_qdrp_batched": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "z2_qdrp_batched_noalloc": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,

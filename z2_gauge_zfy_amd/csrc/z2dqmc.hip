@@ -98,6 +98,7 @@ struct z2_ctx {
   long upd_stride = 0;
   int *nacc = nullptr, *mu_done = nullptr;
   int kdelay = 16;  // delay depth of the flip kernel
+  int kin_cb = 8, kin_rb = 8;  // kinetic tiles: columns per row-rotation block, rows per column-rotation block
   int kdelay_max = 16;
   size_t lds_max = 65536;
   unsigned long long *accc = nullptr;
@@ -615,6 +616,19 @@ extern "C" int z2_set_delay(z2_ctx *c, int32_t k) {
   return Z2_OK;
 }
 
+extern "C" int z2_set_kin_tiles(z2_ctx *c, int32_t rows_cb, int32_t cols_rb) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if ((rows_cb != 8 && rows_cb != 16) || (cols_rb != 8 && cols_rb != 32))
+    return set_err(Z2_EINVAL, "kinetic tiles: rows kernel 8 or 16 columns, columns kernel 8 or 32 rows");
+  // FP64 tiles (the time-displaced path runs the kinetic kernels in FP64 in both modes)
+  const size_t tab = sizeof(int) * 2 * (size_t)c->N;
+  if (8 * (size_t)std::max(rows_cb, cols_rb) * c->N + tab > c->lds_max)
+    return set_err(Z2_EINVAL, "kinetic tile exceeds the LDS at N = %d (FP64 kernels)", c->N);
+  c->kin_cb = rows_cb;
+  c->kin_rb = cols_rb;
+  return Z2_OK;
+}
+
 extern "C" int z2_set_stab_mode(z2_ctx *c, int32_t mode) {
   if (!c || (mode != Z2_STAB_REFERENCE && mode != Z2_STAB_DSPLIT)) return set_err(Z2_EINVAL, "bad stabilization mode");
   c->stab_mode = mode;
@@ -870,33 +884,45 @@ static KinMat<R> kmat(R *m, int nflag, int flush = 0, int pre1 = 0, int pre2 = 0
   return k;
 }
 
-static const int KCB = 8;  // columns per block, row-rotation kernel (8: best measured of 4, 8, 16, 32)
 static constexpr int FLIP_THREADS = 512;  // must not exceed k_flip's __launch_bounds__
-static const int KRB = 8;  // rows per block, column-rotation kernel (8: best measured of 4, 8, 16, 32)
 
-// the kinetic kernels compiled for N = 64, 144, 256 (L = 8, 12, 16), else for any N
-template <typename R, int NC>
-static int launch_rows_n(z2_ctx *c, KinArgs<R> &a, int nmat) {
-  dim3 grid((c->N + KCB - 1) / KCB, c->nch, nmat);
-  size_t sh = sizeof(R) * KCB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+// Kinetic tiles: columns per block of the row-rotation kernel (kin_cb) and rows per
+// block of the column-rotation kernel (kin_rb), chosen per context from the
+// compiled sizes (z2_set_kin_tiles, the autotuner).  Every element sees the same
+// operations in the same order whatever the tile, so G does not depend on it.
+template <typename R, int CB, int NC>
+static int launch_rows_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  dim3 grid((c->N + CB - 1) / CB, c->nch, nmat);
+  size_t sh = sizeof(R) * CB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
   if (sh > 65536)
-    HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, KCB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  hipLaunchKernelGGL((k_kin_rows<R, KCB, NC>), grid, dim3(256), sh, c->stream, a);
+    HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, CB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL((k_kin_rows<R, CB, NC>), grid, dim3(256), sh, c->stream, a);
   LAUNCH_CHECK();
   return Z2_OK;
+}
+
+template <typename R, int RB, int NC>
+static int launch_cols_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  dim3 grid((c->N + RB - 1) / RB, c->nch, nmat);
+  size_t sh = sizeof(R) * RB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+  if (sh > 65536)
+    HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, RB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL((k_kin_cols<R, RB, NC>), grid, dim3(256), sh, c->stream, a);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+template <typename R, int NC>
+static int launch_rows_n(z2_ctx *c, KinArgs<R> &a, int nmat) {
+  return c->kin_cb == 16 ? launch_rows_t<R, 16, NC>(c, a, nmat) : launch_rows_t<R, 8, NC>(c, a, nmat);
 }
 
 template <typename R, int NC>
 static int launch_cols_n(z2_ctx *c, KinArgs<R> &a, int nmat) {
-  dim3 grid((c->N + KRB - 1) / KRB, c->nch, nmat);
-  size_t sh = sizeof(R) * KRB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
-  if (sh > 65536)
-    HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, KRB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  hipLaunchKernelGGL((k_kin_cols<R, KRB, NC>), grid, dim3(256), sh, c->stream, a);
-  LAUNCH_CHECK();
-  return Z2_OK;
+  return c->kin_rb == 32 ? launch_cols_t<R, 32, NC>(c, a, nmat) : launch_cols_t<R, 8, NC>(c, a, nmat);
 }
 
+// the kinetic kernels compiled for N = 64, 144, 256 (L = 8, 12, 16), else for any N
 template <typename R>
 static int launch_rows(z2_ctx *c, KinArgs<R> &a, int nmat) {
   CHK(prof_begin(c, Z2_K_KIN_ROWS));
@@ -1413,8 +1439,18 @@ static int trsm(z2_ctx *c, const double *R, double *X, int right, int n = 0) {
   return Z2_OK;
 }
 
+// device copies as a kernel (every step of a sweep stays a kernel launch, so that
+// a sweep captures into a HIP graph of kernel nodes only)
+__global__ void __launch_bounds__(256) k_dcopy(double *__restrict__ dst, const double *__restrict__ src, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 static int dcopy(z2_ctx *c, double *dst, const double *src, size_t n) {
-  HIPC(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
+  if (!n) return Z2_OK;
+  size_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_dcopy, dim3((unsigned)(nb > 2048 ? 2048 : nb)), dim3(256), 0, c->stream, dst, src, n);
+  LAUNCH_CHECK();
   return Z2_OK;
 }
 
@@ -2236,37 +2272,66 @@ static int run_steps(z2_ctx *c, const std::vector<Step> &v, int *nobst) {
   return Z2_OK;
 }
 
-extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
+// the device work of one LocalSweep%sweep with every chain in direction is_beta:
+// observables reset, the steps, the averages -- kernel launches only (no host
+// synchronisation, no memset), so it can be captured in a HIP graph
+static int sweep_prologue(z2_ctx *c, int toggle) {
+  CHK(z2_obs_reset(c));
+  if (toggle) CHK(dzero(c, c->otau, (size_t)c->nch * TAU_NOBS * c->T * c->N));
+  return Z2_OK;
+}
+
+static int sweep_epilogue(z2_ctx *c, int toggle, int nobst) {
+  CHK(obs_ave(c));
+  if (toggle && nobst > 0) {
+    hipLaunchKernelGGL(k_scale, dim3(256), dim3(256), 0, c->stream, c->otau, (size_t)c->nch * TAU_NOBS * c->T * c->N,
+                       1.0 / (double)nobst);
+    LAUNCH_CHECK();
+  }
+  return Z2_OK;
+}
+
+static int sweep_core(z2_ctx *c, int is_beta, int toggle) {
+  CHK(sweep_prologue(c, toggle));
+  std::vector<Step> v;
+  sweep_steps(c, is_beta, toggle, v);
+  int nobst = 0;
+  CHK(run_steps(c, v, &nobst));
+  return sweep_epilogue(c, toggle, nobst);
+}
+
+static int sweep_check(z2_ctx *c, int is_beta, int toggle) {
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_sweep");
   if (toggle && !c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
   if (is_beta != 0 && is_beta != 1 && is_beta != Z2_BETA_PER_CHAIN)
     return set_err(Z2_EINVAL, "is_beta must be 0, 1 or Z2_BETA_PER_CHAIN");
+  return Z2_OK;
+}
+
+extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
+  CHK(sweep_check(c, is_beta, toggle));
   CHK(h_zero_sync(c));
+  if (is_beta != Z2_BETA_PER_CHAIN) {
+    c->isb.assign(c->nch_all, is_beta);
+    return sweep_core(c, is_beta, toggle);
+  }
+  // each chain in its own order (after Wolff updates the directions differ)
   CHK(z2_obs_reset(c));
   const size_t ntau = (size_t)c->nch * TAU_NOBS * c->T * c->N;
   if (toggle) CHK(dzero(c, c->otau, ntau));
-  int nobst = 0;
-  if (is_beta != Z2_BETA_PER_CHAIN) {
-    c->isb.assign(c->nch_all, is_beta);
+  int n1;
+  CHK(partition(c, &n1));
+  int nob[2] = {0, 0};
+  for (int g = 0; g < 2; ++g) {
+    const int c0 = g == 0 ? 0 : n1, cnt = g == 0 ? n1 : c->nch_all - n1;
+    if (!cnt) continue;
     std::vector<Step> v;
-    sweep_steps(c, is_beta, toggle, v);
-    CHK(run_steps(c, v, &nobst));
-  } else {
-    // each chain in its own order (after Wolff updates the directions differ)
-    int n1;
-    CHK(partition(c, &n1));
-    int nob[2] = {0, 0};
-    for (int g = 0; g < 2; ++g) {
-      const int c0 = g == 0 ? 0 : n1, cnt = g == 0 ? n1 : c->nch_all - n1;
-      if (!cnt) continue;
-      std::vector<Step> v;
-      sweep_steps(c, g == 0 ? 1 : 0, toggle, v);
-      ChainRange r(c, c0, cnt, c->stream);
-      CHK(run_steps(c, v, &nob[g]));
-    }
-    nobst = nob[0] > nob[1] ? nob[0] : nob[1];
+    sweep_steps(c, g == 0 ? 1 : 0, toggle, v);
+    ChainRange r(c, c0, cnt, c->stream);
+    CHK(run_steps(c, v, &nob[g]));
   }
+  const int nobst = nob[0] > nob[1] ? nob[0] : nob[1];
   CHK(obs_ave(c));
   if (toggle && nobst > 0) {
     hipLaunchKernelGGL(k_scale, dim3(256), dim3(256), 0, c->stream, c->otau, ntau, 1.0 / (double)nobst);
@@ -2731,3 +2796,5 @@ extern "C" int z2_profile_work(z2_ctx *c, double *work) {
   *work = c->prof_work;
   return Z2_OK;
 }
+
+#include "batch.hpp"

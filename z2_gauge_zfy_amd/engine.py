@@ -9,7 +9,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._lib import Z2ChainDesc, Z2Params, Z2_BETA_PER_CHAIN, Z2_PREC_FP64, check, lib
+from ._lib import (Z2BatchChain, Z2ChainDesc, Z2Params, Z2_BETA_PER_CHAIN, Z2_EXEC_EAGER, Z2_PREC_FP64, check,
+                   lib)
 
 OBS_NAMES = ("flux_avg", "chi_flux", "pair_sc_q0", "cdw_pi", "cdw_shift", "cdw_ratio", "density")
 TAU_NAMES = ("spin", "pair_s", "pair_d", "charge_s", "charge_d", "single", "curxx")
@@ -249,6 +250,9 @@ class Engine:
     def set_delay(self, k: int):
         check(lib().z2_set_delay(self._ptr, int(k)))
 
+    def set_kin_tiles(self, rows_cb: int, cols_rb: int):
+        check(lib().z2_set_kin_tiles(self._ptr, int(rows_cb), int(cols_rb)))
+
     def set_stab_mode(self, mode: int):
         """Z2_STAB_REFERENCE (default) or Z2_STAB_DSPLIT (D-split G recompute, opt-in)"""
         check(lib().z2_set_stab_mode(self._ptr, int(mode)))
@@ -276,3 +280,128 @@ class Engine:
         s = C.c_void_p()
         check(lib().z2_get_stream(self._ptr, C.byref(s)))
         return s.value or 0
+
+
+class _CtxView(Engine):
+    """an Engine view of a context owned by a Batch (not destroyed by the view)"""
+
+    def __init__(self, ptr, N, T, nch):
+        self._ptr = C.c_void_p(ptr)
+        self._owned = False
+        self.N, self.T, self.nch = N, T, nch
+
+    def close(self):
+        self._ptr = C.c_void_p()
+
+
+@dataclass
+class BatchChain:
+    """one chain of a batch: its own lattice, Ltrot, Nwrap, couplings and seed"""
+    L: int
+    Ltrot: int
+    Nwrap: int
+    spec: ChainSpec = field(default_factory=ChainSpec)
+
+
+class Batch:
+    """Chains of uneven L, Ltrot, Nwrap and couplings, one context per shape group
+    (z2_batch_* in include/z2dqmc.h)."""
+
+    def __init__(self, chains, precision: int = Z2_PREC_FP64, Nsweep: int = 1, max_group: int = 0):
+        self.chains = list(chains)
+        arr = (Z2BatchChain * len(self.chains))()
+        for k, c in enumerate(self.chains):
+            s = c.spec
+            arr[k] = Z2BatchChain(c.L, c.L, c.Ltrot, c.Nwrap,
+                                  Z2ChainDesc(s.RT, s.J, s.h, s.mu, s.Q, s.Beta, int(s.seed), int(s.absolute)))
+        self._ptr = C.c_void_p()
+        check(lib().z2_batch_create(C.cast(arr, C.c_void_p), len(self.chains), precision, Nsweep, max_group,
+                                    C.byref(self._ptr)))
+
+    def close(self):
+        if getattr(self, "_ptr", None) is not None and self._ptr.value:
+            lib().z2_batch_destroy(self._ptr)
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        n, g = C.c_int32(), C.c_int32()
+        check(lib().z2_batch_info(self._ptr, C.byref(n), C.byref(g)))
+        return n.value, g.value
+
+    def where(self, k: int):
+        """(engine view of the chain's group context, index of the chain in it)"""
+        p, i = C.c_void_p(), C.c_int32()
+        check(lib().z2_batch_chain_ctx(self._ptr, k, C.byref(p), C.byref(i)))
+        c = self.chains[k]
+        N, nch, prec = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().z2_ctx_info(p, C.byref(N), C.byref(nch), C.byref(prec)))
+        return _CtxView(p.value, c.L * c.L, c.Ltrot, nch.value), i.value
+
+    def group(self, g: int) -> Engine:
+        p = C.c_void_p()
+        check(lib().z2_batch_group_ctx(self._ptr, g, C.byref(p)))
+        N, nch, prec = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().z2_ctx_info(p, C.byref(N), C.byref(nch), C.byref(prec)))
+        e = _CtxView(p.value, N.value, 0, nch.value)
+        return e
+
+    def init_fields(self):
+        check(lib().z2_batch_init_fields(self._ptr))
+
+    def enable_tau(self):
+        check(lib().z2_batch_enable_tau(self._ptr))
+
+    def pre(self):
+        check(lib().z2_batch_pre(self._ptr))
+
+    def sweep(self, is_beta=True, toggle=False):
+        b = Z2_BETA_PER_CHAIN if is_beta is None else int(bool(is_beta))
+        check(lib().z2_batch_sweep(self._ptr, b, int(bool(toggle))))
+
+    def sync(self):
+        check(lib().z2_batch_sync(self._ptr))
+
+    def set_exec(self, mode: int):
+        check(lib().z2_batch_set_exec(self._ptr, int(mode)))
+
+    def exec_mode(self) -> int:
+        m = C.c_int32()
+        check(lib().z2_batch_get_exec(self._ptr, C.byref(m)))
+        return m.value
+
+    def autotune(self, warm=1, sweeps=1, drift_tol=5e-3, path=None):
+        check(lib().z2_batch_autotune(self._ptr, int(warm), int(sweeps), float(drift_tol),
+                                      path.encode() if path else None))
+
+    def load_tuning(self, path):
+        check(lib().z2_batch_load_tuning(self._ptr, path.encode()))
+
+    # per-chain results through the chain's group context
+    def _call(self, k, name, *a):
+        e, i = self.where(k)
+        return getattr(e, name)(i, *a)
+
+    def sigma(self, k):
+        return self._call(k, "sigma")
+
+    def seed(self, k):
+        return self._call(k, "seed")
+
+    def green(self, k):
+        return self._call(k, "green")
+
+    def counters(self, k):
+        return self._call(k, "counters")
+
+    def obs_equal(self, k):
+        return self._call(k, "obs_equal")
+
+    def drift(self, k):
+        return self._call(k, "drift")
+
