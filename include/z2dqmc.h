@@ -31,7 +31,8 @@ extern "C" {
 #define Z2_ESTATE (-4)   /* call out of order (e.g. sweep before pre) */
 
 #define Z2_PREC_FP64 0   /* everything FP64 (the reference's arithmetic) */
-#define Z2_PREC_MIXED 1  /* FP32 propagation/flips/delayed updates, FP64 QR/UDV/G recompute/ratio */
+#define Z2_PREC_MIXED 1  /* G and the delayed updates in FP32 (transformed in FP64 arithmetic),
+                            FP64 propagators, QR/UDV, G recompute and ratio */
 
 /* shared parameters: the size-defining lines of paramC_sets.txt
  * (reference src/calc_basic.f90:53-61) plus the precision mode */
@@ -133,6 +134,12 @@ int z2_sweep_shadow(z2_ctx *ctx, z2_ctx *ref, int32_t is_beta, double *maxdiff);
  * at the start of the right pass on both contexts, accumulators normalised as
  * in z2_sweep_tau (needs z2_enable_tau on both) */
 int z2_sweep_shadow_tau(z2_ctx *ctx, z2_ctx *ref, int32_t is_beta, int32_t toggle, double *maxdiff);
+/* diagnostic: one shadow sweep recording, after every step, the max over chains
+ * and elements of |G_ctx - G_ref| in trace[i] and the step in code[i]
+ * (kind << 16 | nt; kinds 0 Wrap_L, 1 Wrap_R, 2 Obs_equal%calc, 3 slice of the
+ * left pass, 4 slice of the right pass); *nsteps <= cap steps */
+int z2_sweep_shadow_trace(z2_ctx *ctx, z2_ctx *ref, int32_t is_beta, double *trace, int32_t *code, int32_t cap,
+                          int32_t *nsteps);
 
 /* fine-grained steps, all chains at slice nt (same semantics as the reference
  * routines named): */
@@ -195,6 +202,11 @@ int z2_set_delay(z2_ctx *ctx, int32_t k);
  * large beta where the reference's form loses it; it is a deviation from the
  * reference's arithmetic (G agrees to rounding where both are accurate), for
  * opt-in use.  The time-displaced recompute is not affected. */
+/* mixed mode (no counterpart in the reference): precision of the propagators
+ * UUR, UUL between stabilizations (the B-products the UDV checkpoints are made
+ * from): 64 (FP64, the default) or 32 (FP32, as G; about 1.6% faster at L = 16,
+ * but the recomputed G then carries the products' FP32 error); set before z2_pre */
+int z2_set_u_precision(z2_ctx *ctx, int32_t bits);
 /* kinetic tiles (no counterpart in the reference): columns per workgroup of the
  * row-rotation kernel (8 or 16) and rows per workgroup of the column-rotation
  * kernel (8 or 32); results do not depend on them (default 8, 8) */

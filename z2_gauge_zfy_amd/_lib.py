@@ -72,6 +72,8 @@ _SIGS = {
     "z2_sweep_shadow": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_sweep_shadow_tau": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32,
                                       C.POINTER(C.c_double)]),
+    "z2_sweep_shadow_trace": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_int32)]),
     "z2_obs_equal_ave": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_sweep_reset": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_prop_K_pre": (C.c_int, [C.c_void_p, C.c_int32]),
@@ -118,6 +120,7 @@ _SIGS = {
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "z2_profile_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     "z2_set_kin_tiles": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "z2_set_u_precision": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                   C.POINTER(C.c_void_p)]),
     "z2_batch_destroy": (C.c_int, [C.c_void_p]),

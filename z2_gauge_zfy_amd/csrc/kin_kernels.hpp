@@ -37,17 +37,34 @@ struct KinArgs {
   LatticeDev lat;
 };
 
+// Arithmetic: the matrices are stored in R (FP64, or FP32 in the mixed mode) and
+// transformed in FP64.  A stored element is rounded to R once per procedure of
+// the reference: the flush of the pending updates and the 8 rotation stages run
+// in FP64 on the loaded value; each scalar mu factor (propT_*, a procedure of its
+// own) is applied as the product in R, so a fused launch rounds exactly where the
+// reference's procedure sequence does.  For R = double nothing is rounded beyond
+// the FP64 operations themselves.
 template <typename R>
-__device__ __forceinline__ R apply_fac(R x, int sel, const ChainConst &c) {
-  if (sel == FAC_EP) return x * (R)c.ep;
-  if (sel == FAC_EM) return x * (R)c.em;
+__device__ __forceinline__ double to_r(double x) {
+  return (double)(R)x;
+}
+
+// x times the scalar factor sel, as the product in R (exact in FP64 before the
+// rounding: both operands have R's precision)
+template <typename R>
+__device__ __forceinline__ double apply_fac(double x, int sel, const ChainConst &c) {
+  if (sel == FAC_EP) return to_r<R>(x * (double)(R)c.ep);
+  if (sel == FAC_EM) return to_r<R>(x * (double)(R)c.em);
   return x;
 }
 
-// Prologue applied to element (r, col) of a chain's matrix.
+// Prologue applied to element (r, col) of a chain's matrix: scalar factors, then
+// the pending updates in acceptance order (FP64), then scalar factors on the
+// rounded result.
 template <typename R>
-__device__ __forceinline__ R kin_prologue(R x, int r, int col, const KinMat<R> &km,
-                                          const KinArgs<R> &a, const ChainConst &c, int chain) {
+__device__ __forceinline__ double kin_prologue(R xin, int r, int col, const KinMat<R> &km,
+                                               const KinArgs<R> &a, const ChainConst &c, int chain) {
+  double x = (double)xin;
   if (!(km.pre_cond && a.mu_done[chain])) {
     x = apply_fac<R>(x, km.pre1, c);
     x = apply_fac<R>(x, km.pre2, c);
@@ -58,17 +75,31 @@ __device__ __forceinline__ R kin_prologue(R x, int r, int col, const KinMat<R> &
     const R *T = a.Tbuf + chain * a.upd_stride;
     const R *V = a.Vbuf + chain * a.upd_stride;
     for (int k = 0; k < na; ++k) {
-      R t0 = T[(2 * k) * N + r], t1 = T[(2 * k + 1) * N + r];
-      R v0 = V[(2 * k) * N + col], v1 = V[(2 * k + 1) * N + col];
-      R p = t0 * v0;
-      R q = t1 * v1;
-      R d = p + q;
+      double t0 = (double)T[(2 * k) * N + r], t1 = (double)T[(2 * k + 1) * N + r];
+      double v0 = (double)V[(2 * k) * N + col], v1 = (double)V[(2 * k + 1) * N + col];
+      double p = t0 * v0;
+      double q = t1 * v1;
+      double d = p + q;
       x = x - d;
     }
   }
-  x = apply_fac<R>(x, km.mid1, c);
-  x = apply_fac<R>(x, km.mid2, c);
+  if (km.mid1 || km.mid2) {
+    x = to_r<R>(x);
+    x = apply_fac<R>(x, km.mid1, c);
+    x = apply_fac<R>(x, km.mid2, c);
+  }
   return x;
+}
+
+// the stored value of a transformed element with the epilogue factors
+template <typename R>
+__device__ __forceinline__ R kin_epilogue(double x, const KinMat<R> &km, const ChainConst &c) {
+  if (km.post || km.post2) {
+    x = to_r<R>(x);
+    x = apply_fac<R>(x, km.post, c);
+    x = apply_fac<R>(x, km.post2, c);
+  }
+  return (R)x;
 }
 
 // Rotation table of one slice in LDS: for each of the 4 checkerboard groups and
@@ -90,7 +121,7 @@ __device__ __forceinline__ int kin_tab_entry(const LatticeDev &lat, const int8_t
 template <typename R, int CB, int NC = 0>
 __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
-  R *tile = reinterpret_cast<R *>(smem_raw);                   // [CB][N]
+  double *tile = reinterpret_cast<double *>(smem_raw);         // [CB][N]
   int *tab = reinterpret_cast<int *>(tile + (size_t)CB * a.lat.N);  // [4][N/2]
   const int N = NC ? NC : a.lat.N, N2 = 2 * N, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
   const int chain = blockIdx.y;
@@ -108,7 +139,7 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
     for (int r = tid; r < N; r += nt) tile[cc * N + r] = kin_prologue<R>(M[r + (size_t)(col0 + cc) * N], r, col0 + cc, km, a, c, chain);
   __syncthreads();
   if (km.nflag != 0) {
-    const R Cp = (R)c.Cp, Cm = (R)c.Cm, Sp = (R)c.Sp, Sm = (R)c.Sm;
+    const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
     const int ngrp = half >= nt ? 1 : nt / half;  // column groups
     const int items = half * ngrp;
     for (int st = 0; st < 8; ++st) {
@@ -117,11 +148,11 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
         const int k = it % half, cg = it / half;
         const int e = t[k];
         const int p1 = e & 2047, p2 = (e >> 11) & 2047;
-        const R C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
+        const double C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
         for (int cc = cg; cc < ncol; cc += ngrp) {
-          R x1 = tile[cc * N + p1], x2 = tile[cc * N + p2];
-          R v1 = C * x1 + S * x2;
-          R v2 = S * x1 + C * x2;
+          double x1 = tile[cc * N + p1], x2 = tile[cc * N + p2];
+          double v1 = C * x1 + S * x2;
+          double v2 = S * x1 + C * x2;
           tile[cc * N + p1] = v1;
           tile[cc * N + p2] = v2;
         }
@@ -130,8 +161,7 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
     }
   }
   for (int cc = 0; cc < ncol; ++cc)
-    for (int r = tid; r < N; r += nt)
-      M[r + (size_t)(col0 + cc) * N] = apply_fac<R>(apply_fac<R>(tile[cc * N + r], km.post, c), km.post2, c);
+    for (int r = tid; r < N; r += nt) M[r + (size_t)(col0 + cc) * N] = kin_epilogue<R>(tile[cc * N + r], km, c);
 }
 
 // Mat <- Mat B^{nflag} : each row is independent; a block owns RB rows.  Thread
@@ -139,7 +169,7 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
 template <typename R, int RB, int NC = 0>
 __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
   extern __shared__ unsigned char smem_raw[];
-  R *tile = reinterpret_cast<R *>(smem_raw);                   // [N][RB]
+  double *tile = reinterpret_cast<double *>(smem_raw);         // [N][RB]
   int *tab = reinterpret_cast<int *>(tile + (size_t)RB * a.lat.N);  // [4][N/2]
   const int N = NC ? NC : a.lat.N, N2 = 2 * N, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
   const int chain = blockIdx.y;
@@ -159,17 +189,17 @@ __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
       tile[col * RB + r] = kin_prologue<R>(M[(row0 + r) + (size_t)col * N], row0 + r, col, km, a, c, chain);
   __syncthreads();
   if (km.nflag != 0) {
-    const R Cp = (R)c.Cp, Cm = (R)c.Cm, Sp = (R)c.Sp, Sm = (R)c.Sm;
+    const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
     for (int st = 0; st < 8; ++st) {
       const int *t = tab + kin_group_of_stage(st) * half;
       if (r < nrow)
         for (int k = tid / RB; k < half; k += cstep) {
           const int e = t[k];
           const int p1 = e & 2047, p2 = (e >> 11) & 2047;
-          const R C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
-          R x1 = tile[p1 * RB + r], x2 = tile[p2 * RB + r];
-          R u1 = x1 * C + x2 * S;
-          R u2 = x1 * S + x2 * C;
+          const double C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
+          double x1 = tile[p1 * RB + r], x2 = tile[p2 * RB + r];
+          double u1 = x1 * C + x2 * S;
+          double u2 = x1 * S + x2 * C;
           tile[p1 * RB + r] = u1;
           tile[p2 * RB + r] = u2;
         }
@@ -178,7 +208,7 @@ __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
   }
   if (r < nrow)
     for (int col = tid / RB; col < N; col += cstep)
-      M[(row0 + r) + (size_t)col * N] = apply_fac<R>(apply_fac<R>(tile[col * RB + r], km.post, c), km.post2, c);
+      M[(row0 + r) + (size_t)col * N] = kin_epilogue<R>(tile[col * RB + r], km, c);
 }
 
 // Elementwise prologue only (flush and/or mu factors) over whole matrices.
@@ -192,6 +222,6 @@ __global__ void __launch_bounds__(256) k_kin_elem(KinArgs<R> a) {
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N;
        idx += gridDim.x * blockDim.x) {
     int col = idx / N, r = idx - col * N;
-    M[idx] = apply_fac<R>(apply_fac<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km.post, c), km.post2, c);
+    M[idx] = kin_epilogue<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km, c);
   }
 }

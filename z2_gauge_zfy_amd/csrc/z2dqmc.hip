@@ -118,6 +118,7 @@ struct z2_ctx {
   double *dsc = nullptr;  // D-split scales [chain][4][N]
   // mixed precision: propagated G, UUR, UUL and the delayed updates in FP32
   bool mixed = false;
+  bool u64 = false;  // mixed mode: the right/left propagators UUR, UUL (the B-products) kept in FP64
   float *Gf = nullptr, *UURf = nullptr, *UULf = nullptr, *Tbf = nullptr, *Vbf = nullptr;
   // decisions of the last flip launch (bit k = proposal k), and decisions to replay
   uint32_t *rec = nullptr;
@@ -538,6 +539,7 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     if ((rc = dalloc(c, &c->rec, B * c->dwords))) break;
     if ((rc = dalloc(c, &c->cmp, B))) break;
     c->mixed = p->precision == Z2_PREC_MIXED;
+    c->u64 = c->mixed;  // the B-products in FP64 by default (z2_set_u_precision)
     if (c->mixed) {
       if ((rc = dalloc(c, &c->Gf, B * nn))) break;
       if ((rc = dalloc(c, &c->UURf, B * nn))) break;
@@ -613,6 +615,14 @@ extern "C" int z2_set_delay(z2_ctx *c, int32_t k) {
   if (!c || k < 1 || k > c->kdelay_max)
     return set_err(Z2_EINVAL, "delay depth must be in [1, %d] for N = %d", c ? c->kdelay_max : 0, c ? c->N : 0);
   c->kdelay = k;
+  return Z2_OK;
+}
+
+extern "C" int z2_set_u_precision(z2_ctx *c, int32_t bits) {
+  if (!c || (bits != 32 && bits != 64)) return set_err(Z2_EINVAL, "propagator precision must be 32 or 64 bits");
+  if (!c->mixed) return bits == 64 ? Z2_OK : set_err(Z2_EINVAL, "FP64 contexts propagate in FP64");
+  if (c->have_pre) return set_err(Z2_ESTATE, "set the propagator precision before z2_pre");
+  c->u64 = bits == 64;
   return Z2_OK;
 }
 
@@ -893,7 +903,7 @@ static constexpr int FLIP_THREADS = 512;  // must not exceed k_flip's __launch_b
 template <typename R, int CB, int NC>
 static int launch_rows_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
   dim3 grid((c->N + CB - 1) / CB, c->nch, nmat);
-  size_t sh = sizeof(R) * CB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+  size_t sh = sizeof(double) * CB * c->N + sizeof(int) * 2 * c->N;  // FP64 tile + rotation table
   if (sh > 65536)
     HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, CB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   hipLaunchKernelGGL((k_kin_rows<R, CB, NC>), grid, dim3(256), sh, c->stream, a);
@@ -904,7 +914,7 @@ static int launch_rows_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
 template <typename R, int RB, int NC>
 static int launch_cols_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
   dim3 grid((c->N + RB - 1) / RB, c->nch, nmat);
-  size_t sh = sizeof(R) * RB * c->N + sizeof(int) * 2 * c->N;  // tile + rotation table
+  size_t sh = sizeof(double) * RB * c->N + sizeof(int) * 2 * c->N;  // FP64 tile + rotation table
   if (sh > 65536)
     HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, RB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   hipLaunchKernelGGL((k_kin_cols<R, RB, NC>), grid, dim3(256), sh, c->stream, a);
@@ -1561,7 +1571,7 @@ extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
   if (nt != 0) {
-    if (c->mixed) CHK(cvt(c, (const float *)c->UURf, c->UUR));
+    if (c->mixed && !c->u64) CHK(cvt(c, (const float *)c->UURf, c->UUR));
     CHK(stab_UR(c));
   }
   CHK(dcopy(c, c->URl + slot(c, st), c->UUR, nn));
@@ -1576,7 +1586,7 @@ extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
     c->have_pre = true;
     c->left_done = 0;
   }
-  if (c->mixed) CHK(cvt(c, (const double *)c->UUR, c->UURf));
+  if (c->mixed && !c->u64) CHK(cvt(c, (const double *)c->UUR, c->UURf));
   return Z2_OK;
 }
 
@@ -1588,7 +1598,7 @@ extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
   CHK(dcopy(c, c->VUR, c->VRl + slot(c, st), nn));
   CHK(dcopy(c, c->DUR, c->DRl + slotd(c, st), nd));
   if (nt != c->T) {
-    if (c->mixed) CHK(cvt(c, (const float *)c->UULf, c->UUL));
+    if (c->mixed && !c->u64) CHK(cvt(c, (const float *)c->UULf, c->UUL));
     CHK(stab_UL(c));
     CHK(stab_green(c, nt));
     CHK(drift_replace(c, flag));
@@ -1596,7 +1606,7 @@ extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
   CHK(dcopy(c, c->ULl + slot(c, st), c->UUL, nn));
   CHK(dcopy(c, c->VLl + slot(c, st), c->VUL, nn));
   CHK(dcopy(c, c->DLl + slotd(c, st), c->DUL, nd));
-  if (c->mixed) CHK(cvt(c, (const double *)c->UUL, c->UULf));
+  if (c->mixed && !c->u64) CHK(cvt(c, (const double *)c->UUL, c->UULf));
   if (nt == 0) c->left_done = c->nch == c->nch_all ? 1 : -1;
   return Z2_OK;
 }
@@ -1609,7 +1619,7 @@ extern "C" int z2_wrap_R(z2_ctx *c, int32_t nt, int32_t flag) {
   CHK(dcopy(c, c->VUL, c->VLl + slot(c, st), nn));
   CHK(dcopy(c, c->DUL, c->DLl + slotd(c, st), nd));
   if (nt != 0) {
-    if (c->mixed) CHK(cvt(c, (const float *)c->UURf, c->UUR));
+    if (c->mixed && !c->u64) CHK(cvt(c, (const float *)c->UURf, c->UUR));
     CHK(stab_UR(c));
     CHK(stab_green(c, nt));
     CHK(drift_replace(c, flag));
@@ -1617,21 +1627,40 @@ extern "C" int z2_wrap_R(z2_ctx *c, int32_t nt, int32_t flag) {
   CHK(dcopy(c, c->URl + slot(c, st), c->UUR, nn));
   CHK(dcopy(c, c->VRl + slot(c, st), c->VUR, nn));
   CHK(dcopy(c, c->DRl + slotd(c, st), c->DUR, nd));
-  if (c->mixed) CHK(cvt(c, (const double *)c->UUR, c->UURf));
+  if (c->mixed && !c->u64) CHK(cvt(c, (const double *)c->UUR, c->UURf));
   if (nt == c->T) c->left_done = c->nch == c->nch_all ? 0 : -1;
   return Z2_OK;
 }
 
 // ---------------------------------------------------------------- slices
 // propT_L + LocalK_prop_L (src/multiply.f90:79-93, src/localK.f90:347-373)
+// the propagators UUR, UUL in the precision of G (one launch with G), or in FP64
+// while G is FP32 (mixed mode with FP64 B-products)
+template <typename R>
+static bool u_with_g(const z2_ctx *c) {
+  return sizeof(R) == 8 || !c->u64;
+}
+
+// U <- U B^{nflag} (cols) or B^{nflag} U (rows) in FP64, with scalar factors
+static int kin_u64(z2_ctx *c, int nt, double *U, int cols, int nflag, int pre, int post) {
+  KinArgs<double> d = kin_args<double>(c, nt);
+  d.m[0] = kmat<double>(U, nflag, 0, pre, 0, 0, 0, post);
+  return cols ? launch_cols<double>(c, d, 1) : launch_rows<double>(c, d, 1);
+}
+
 template <typename R>
 static int slice_L_impl(z2_ctx *c, int nt) {
   CHK(launch_flip<R>(c, nt, 1, 1));
   KinArgs<R> a = kin_args<R>(c, nt);
   a.m[0] = kmat<R>(PB<R>::G(c), +1, 1, FAC_EP, FAC_EM);  // G <- (flush((G*ep)*em)) B
   a.m[0].pre_cond = 1;                                    // mu already in G if the flip kernel flushed
-  a.m[1] = kmat<R>(PB<R>::UUL(c), +1, 0, FAC_EP);        // UUL <- (UUL*ep) B
-  CHK(launch_cols<R>(c, a, 2));
+  if (u_with_g<R>(c)) {
+    a.m[1] = kmat<R>(PB<R>::UUL(c), +1, 0, FAC_EP);      // UUL <- (UUL*ep) B
+    CHK(launch_cols<R>(c, a, 2));
+  } else {
+    CHK(launch_cols<R>(c, a, 1));
+    CHK(kin_u64(c, nt, c->UUL, 1, +1, FAC_EP, 0));
+  }
   a.m[0] = kmat<R>(PB<R>::G(c), -1);                     // G <- B^-1 G
   CHK(launch_rows<R>(c, a, 1));
   return Z2_OK;
@@ -1656,6 +1685,7 @@ static int slice_R_impl(z2_ctx *c, int nt) {
     a.m[0] = kmat<R>(PB<R>::G(c), 0, 1, 0, 0, FAC_EP, FAC_EM);
     CHK(launch_elem_kin<R>(c, a, 1));
   }
+  if (!u_with_g<R>(c)) return kin_u64(c, nt, c->UUR, 0, +1, 0, FAC_EP);
   a.m[0] = kmat<R>(PB<R>::UUR(c), +1, 0, 0, 0, 0, 0, FAC_EP);  // UUR <- (B UUR)*ep
   CHK(launch_rows<R>(c, a, 1));
   return Z2_OK;
@@ -1673,7 +1703,7 @@ extern "C" int z2_slice_R(z2_ctx *c, int32_t nt) {
 
 // propK_pre + propT_pre for all chains at slice nt (src/multiply.f90:8-15,65-77)
 static int pre_prop(z2_ctx *c, int nt) {
-  if (c->mixed) {
+  if (c->mixed && !c->u64) {
     KinArgs<float> a = kin_args<float>(c, nt);
     a.m[0] = kmat<float>(c->UURf, +1, 0, 0, 0, 0, 0, FAC_EP);
     return launch_rows<float>(c, a, 1);
@@ -1915,7 +1945,7 @@ static int check_slice(z2_ctx *c, int nt) {
 // propK_pre, src/multiply.f90:8-15: UUR <- B(nt) UUR
 extern "C" int z2_prop_K_pre(z2_ctx *c, int32_t nt) {
   CHK(check_slice(c, nt));
-  if (c->mixed) {
+  if (c->mixed && !c->u64) {
     KinArgs<float> a = kin_args<float>(c, nt);
     a.m[0] = kmat<float>(c->UURf, +1);
     return launch_rows<float>(c, a, 1);
@@ -1928,13 +1958,17 @@ extern "C" int z2_prop_K_pre(z2_ctx *c, int32_t nt) {
 // propT_pre, src/multiply.f90:65-77: UUR <- UUR exp(dtau mu)
 extern "C" int z2_prop_T_pre(z2_ctx *c, int32_t nt) {
   CHK(check_slice(c, nt));
-  if (c->mixed) return mu_elem<float>(c, nt, c->UURf, FAC_EP, FAC_NONE, (float *)nullptr, 0);
+  if (c->mixed && !c->u64) return mu_elem<float>(c, nt, c->UURf, FAC_EP, FAC_NONE, (float *)nullptr, 0);
   return mu_elem<double>(c, nt, c->UUR, FAC_EP, FAC_NONE, (double *)nullptr, 0);
 }
 
 // propT_L, src/multiply.f90:79-93: G <- (G e^{+}) e^{-}, UUL <- UUL e^{+}
 extern "C" int z2_prop_T_L(z2_ctx *c, int32_t nt) {
   CHK(check_slice(c, nt));
+  if (c->mixed && c->u64) {
+    CHK(mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, (float *)nullptr, 0));
+    return mu_elem<double>(c, nt, c->UUL, FAC_EP, FAC_NONE, (double *)nullptr, 0);
+  }
   if (c->mixed) return mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, c->UULf, FAC_EP);
   return mu_elem<double>(c, nt, c->G, FAC_EP, FAC_EM, c->UUL, FAC_EP);
 }
@@ -1942,6 +1976,10 @@ extern "C" int z2_prop_T_L(z2_ctx *c, int32_t nt) {
 // propT_R, src/multiply.f90:95-108: G <- (G e^{+}) e^{-}, UUR <- UUR e^{+}
 extern "C" int z2_prop_T_R(z2_ctx *c, int32_t nt) {
   CHK(check_slice(c, nt));
+  if (c->mixed && c->u64) {
+    CHK(mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, (float *)nullptr, 0));
+    return mu_elem<double>(c, nt, c->UUR, FAC_EP, FAC_NONE, (double *)nullptr, 0);
+  }
   if (c->mixed) return mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, c->UURf, FAC_EP);
   return mu_elem<double>(c, nt, c->G, FAC_EP, FAC_EM, c->UUR, FAC_EP);
 }
@@ -1953,8 +1991,13 @@ static int localK_L_impl(z2_ctx *c, int nt) {
   CHK(launch_flip<R>(c, nt, 1, 0));
   KinArgs<R> a = kin_args<R>(c, nt);
   a.m[0] = kmat<R>(PB<R>::G(c), +1, 1);
-  a.m[1] = kmat<R>(PB<R>::UUL(c), +1);
-  CHK(launch_cols<R>(c, a, 2));
+  if (u_with_g<R>(c)) {
+    a.m[1] = kmat<R>(PB<R>::UUL(c), +1);
+    CHK(launch_cols<R>(c, a, 2));
+  } else {
+    CHK(launch_cols<R>(c, a, 1));
+    CHK(kin_u64(c, nt, c->UUL, 1, +1, 0, 0));
+  }
   a.m[0] = kmat<R>(PB<R>::G(c), -1);
   return launch_rows<R>(c, a, 1);
 }
@@ -1974,6 +2017,7 @@ static int localK_R_impl(z2_ctx *c, int nt) {
     a.m[0] = kmat<R>(PB<R>::G(c), 0, 1);
     CHK(launch_elem_kin<R>(c, a, 1));
   }
+  if (!u_with_g<R>(c)) return kin_u64(c, nt, c->UUR, 0, +1, 0, 0);
   a.m[0] = kmat<R>(PB<R>::UUR(c), +1);
   return launch_rows<R>(c, a, 1);
 }
@@ -2550,6 +2594,47 @@ extern "C" int z2_sweep_shadow_tau(z2_ctx *c, z2_ctx *ref, int32_t is_beta, int3
 
 extern "C" int z2_sweep_shadow(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *maxdiff_out) {
   return z2_sweep_shadow_tau(c, ref, is_beta, 0, maxdiff_out);
+}
+
+// The deviation's course through one shadow sweep: after every step, max over the
+// chains and elements of |G_ctx - G_ref| (and the step: kind << 16 | nt, kinds
+// 0 Wrap_L, 1 Wrap_R, 2 Obs_equal, 3 slice L, 4 slice R); a diagnostic with a host
+// synchronisation per step.
+extern "C" int z2_sweep_shadow_trace(z2_ctx *c, z2_ctx *ref, int32_t is_beta, double *trace, int32_t *code,
+                                     int32_t cap, int32_t *nsteps) {
+  if (!c || !ref || !trace || !code || !nsteps || cap < 1) return set_err(Z2_EINVAL, "null argument");
+  if (ref->mixed || c->N != ref->N || c->T != ref->T || c->w != ref->w || c->nch != ref->nch ||
+      c->p.Nsweep != ref->p.Nsweep)
+    return set_err(Z2_EINVAL, "shadow needs an FP64 reference context of the same shape");
+  if (!c->have_pre || !ref->have_pre) return set_err(Z2_ESTATE, "z2_pre must run on both contexts");
+  if (c->phys != ref->phys) return set_err(Z2_ESTATE, "shadow needs both contexts' chains in the same storage order");
+  if (is_beta != 0 && is_beta != 1) return set_err(Z2_EINVAL, "shadow sweeps take is_beta 0 or 1");
+  for (z2_ctx *x : {c, ref}) CHK(z2_sweep_reset(x, 0));
+  std::vector<Step> v;
+  sweep_steps(ref, is_beta, 0, v);
+  if ((int)v.size() > cap) return set_err(Z2_EINVAL, "trace holds %d steps, the sweep has %zu", cap, v.size());
+  c->force = ref->rec;
+  int rc = Z2_OK;
+  std::vector<double> m(c->nch);
+  int i = 0;
+  for (const Step &st : v) {
+    if ((rc = run_step(ref, st))) break;
+    HIPC(hipStreamSynchronize(ref->stream));
+    if ((rc = run_step(c, st))) break;
+    if ((rc = dzero(c, c->cmp, (size_t)c->nch))) break;
+    rc = c->mixed ? maxdiff<float>(ref, c, c->Gf) : maxdiff<double>(ref, c, c->G);
+    if (rc) break;
+    HIPC(hipMemcpyAsync(m.data(), c->cmp, sizeof(double) * c->nch, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    trace[i] = *std::max_element(m.begin(), m.end());
+    code[i] = ((int)st.kind << 16) | st.nt;
+    ++i;
+  }
+  c->force = nullptr;
+  CHK(rc);
+  for (z2_ctx *x : {c, ref}) CHK(obs_ave(x));
+  *nsteps = i;
+  return Z2_OK;
 }
 
 // ---------------------------------------------------------------- batched QDRP

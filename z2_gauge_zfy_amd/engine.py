@@ -142,6 +142,16 @@ class Engine:
         check(lib().z2_sweep_shadow_tau(self._ptr, ref.ptr, int(is_beta), int(bool(toggle)), _pd(out)))
         return out
 
+    def sweep_shadow_trace(self, ref: "Engine", is_beta: bool = True):
+        """one shadow sweep; per step: (kind, nt, max over chains of max|G_self - G_ref|)"""
+        cap = 8 * self.T + 64
+        tr = np.zeros(cap)
+        code = np.zeros(cap, np.int32)
+        n = C.c_int32()
+        check(lib().z2_sweep_shadow_trace(self._ptr, ref.ptr, int(is_beta), _pd(tr),
+                                          code.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(n)))
+        return [(int(c) >> 16, int(c) & 0xFFFF, float(t)) for c, t in zip(code[:n.value], tr[:n.value])]
+
     # ---- fine-grained steps (reference procedure granularity), all chains at slice nt
     def _step(self, name, *args):
         check(getattr(lib(), name)(self._ptr, *[int(x) for x in args]))
@@ -249,6 +259,9 @@ class Engine:
 
     def set_delay(self, k: int):
         check(lib().z2_set_delay(self._ptr, int(k)))
+
+    def set_u_precision(self, bits: int):
+        check(lib().z2_set_u_precision(self._ptr, int(bits)))
 
     def set_kin_tiles(self, rows_cb: int, cols_rb: int):
         check(lib().z2_set_kin_tiles(self._ptr, int(rows_cb), int(cols_rb)))
