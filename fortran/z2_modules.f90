@@ -57,8 +57,8 @@ contains
         integer, allocatable :: seeds(:)
         integer :: k, ios, s, conf0
         call read_input()
-        if (NlxTherm /= Nlx .or. NlyTherm /= Nly .or. LtrotTherm /= Ltrot) then
-            write(6, *) "thermalisation-size transfer is not supported by this driver"; stop 1
+        if (LtrotTherm > Ltrot) then
+            write(6, *) "LtrotTherm must not exceed Ltrot (conf_transfer tiles the thermalised field)"; stop 1
         endif
         open(unit=30, file='confin.txt', status='old')
         read(30, *) conf0
@@ -87,21 +87,28 @@ contains
         ctx_params = p
         ctx_chains = d
         if (conf0 == 0) then
-            call z2_check(z2_init_fields(ctx), "z2_init_fields")
+            if (NlxTherm == Nlx .and. NlyTherm == Nly .and. LtrotTherm == Ltrot) then
+                call z2_check(z2_init_fields(ctx), "z2_init_fields")
+            else  ! conf_set at the thermalisation size, then conf_transfer (src/fields.f90:429-507)
+                call z2_check(z2_init_fields_therm(ctx, int(NlxTherm, c_int32_t), int(NlyTherm, c_int32_t), &
+                              int(LtrotTherm, c_int32_t)), "z2_init_fields_therm")
+            endif
         else
             call conf_restart()
         endif
         call z2_check(z2_get_seed(ctx, 0_c_int32_t, iseed), "z2_get_seed")
     end subroutine Model_init
 
-    ! restart from confin.txt (conf_therm_in, src/fields.f90:227-309, at the run size):
-    ! per chain its seed, sigma(ii, nt) with nt outer, then lambda(1:Lq); a missing
-    ! lambda is drawn from the chain's seed (lambda_random_fill, src/fields.f90:559-603)
+    ! restart from confin.txt (conf_therm_in, src/fields.f90:227-309): per chain its
+    ! seed, sigma(ii, nt) with nt outer, then lambda(1:Lq), at the thermalisation
+    ! size; a missing lambda is drawn from the chain's seed (lambda_random_fill,
+    ! src/fields.f90:559-603); conf_transfer tiles them onto the run's lattice
     subroutine conf_restart()
         real(c_double), allocatable :: sig(:), lam(:)
-        integer :: k, i, ios
+        integer :: k, i, ios, LqT
         integer(c_int32_t) :: s
-        allocate(sig(2 * Lq * Ltrot), lam(Lq))
+        LqT = NlxTherm * NlyTherm
+        allocate(sig(2 * LqT * LtrotTherm), lam(LqT))
         open(unit=30, file='confin.txt', status='old')
         do k = 0, n_chains - 1
             read(30, *) s
@@ -109,13 +116,13 @@ contains
                 read(30, *) sig(i)
             enddo
             ios = 0
-            do i = 1, Lq
+            do i = 1, LqT
                 read(30, *, iostat=ios) lam(i)
                 if (ios /= 0) exit
             enddo
             if (ios /= 0) call lambda_fill(lam, s)
-            call z2_check(z2_set_sigma(ctx, int(k, c_int32_t), sig, s), "z2_set_sigma")
-            call z2_check(z2_set_lambda(ctx, int(k, c_int32_t), lam), "z2_set_lambda")
+            call z2_check(z2_set_sigma_therm(ctx, int(k, c_int32_t), int(NlxTherm, c_int32_t), &
+                          int(NlyTherm, c_int32_t), int(LtrotTherm, c_int32_t), sig, lam, s), "conf_transfer")
         enddo
         close(30)
     end subroutine conf_restart

@@ -295,6 +295,20 @@ module z2dqmc_c
             real(c_double), intent(out) :: xmax(3), xmean(3)
             integer(c_int) :: rc
         end function
+        function z2_init_fields_therm(ctx, lx, ly, lt) bind(C, name="z2_init_fields_therm") result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: lx, ly, lt
+            integer(c_int) :: rc
+        end function
+        function z2_set_sigma_therm(ctx, chain, lx, ly, lt, sigma, lam, seed) bind(C, name="z2_set_sigma_therm") &
+                result(rc)
+            import :: c_int, c_int32_t, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            integer(c_int32_t), value :: chain, lx, ly, lt, seed
+            real(c_double), intent(in) :: sigma(*), lam(*)
+            integer(c_int) :: rc
+        end function
         function z2_therm(ctx) bind(C, name="z2_therm") result(rc)
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx

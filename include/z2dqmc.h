@@ -72,6 +72,15 @@ int z2_get_seed(z2_ctx *ctx, int32_t chain, int32_t *seed);
  * confin / confout files; the sweeps do not use it */
 int z2_get_lambda(z2_ctx *ctx, int32_t chain, double *lambda);
 int z2_set_lambda(z2_ctx *ctx, int32_t chain, const double *lambda);
+/* thermalisation-size configurations (NlxTherm, NlyTherm, LtrotTherm of
+ * paramC_sets.txt): conf_transfer (src/fields.f90:429-507) tiles a field of that
+ * size (sigma(ii, nt), ii fastest) and its lambda onto the chain's lattice
+ * (LtrotTherm <= Ltrot), sets seed; z2_init_fields_therm is the fresh start at
+ * that size (conf_set there with each chain's seed and absolute mode, then
+ * conf_transfer, src/fields.f90:227-309) */
+int z2_set_sigma_therm(z2_ctx *ctx, int32_t chain, int32_t NlxTherm, int32_t NlyTherm, int32_t LtrotTherm,
+                       const double *sigma_therm, const double *lambda_therm, int32_t seed);
+int z2_init_fields_therm(z2_ctx *ctx, int32_t NlxTherm, int32_t NlyTherm, int32_t LtrotTherm);
 
 /* LocalSweep%pre (src/local_sweep.f90:87-104) */
 int z2_pre(z2_ctx *ctx);

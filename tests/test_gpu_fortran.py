@@ -223,3 +223,33 @@ def test_fortran_fine_grained_sequence_is_bit_identical(tmp_path, precision):
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "bit-identical over 3 bins" in r.stdout
+
+
+def test_fortran_driver_thermalisation_size(tmp_path):
+    """paramC_sets.txt with a thermalisation size (2 x 2 sites, 10 slices) below the
+    run size (4 x 4, 20): the driver starts every chain by conf_set at that size
+    and conf_transfer, and its confout.txt follows the oracle chain made the same way"""
+    if not os.path.exists(EXE):
+        pytest.skip("fortran driver not built")
+    seeds = [777, 777 + 2 * 7919]
+    (tmp_path / "paramC_sets.txt").write_text(
+        "1.0 1.0 0.5 0.0 1\n4 4 20 2.0\n2 2 10\n5 2 1\n.false. 0\n.false. 0\n.false. 0\n1\n")
+    (tmp_path / "seeds.txt").write_text("\n".join(str(s) for s in seeds) + "\n")
+    (tmp_path / "confin.txt").write_text("0\n")
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    p = O.make_params(RT=1.0, J=1.0, h=0.5, mu=0.0, Nlx=4, Nly=4, Ltrot=20, Beta=2.0, Nwrap=5,
+                      NlxTherm=2, NlyTherm=2, LtrotTherm=10)
+    m = O.Model(p)
+    conf = (tmp_path / "confout.txt").read_text().split()
+    n = 1 + 2 * 16 * 20 + 16
+    for k, s in enumerate(seeds):
+        c = O.Chain(m, s)
+        c.pre()
+        for _ in range(2):
+            c.sweep()
+        blk = conf[k * n:(k + 1) * n]
+        assert int(blk[0]) == c.seed
+        assert np.array_equal(np.array([float(x) for x in blk[1:1 + 640]]), c.sigma().reshape(-1))
+        assert np.array_equal(np.array([float(x) for x in blk[1 + 640:]]), c.lam())

@@ -196,3 +196,23 @@ def test_mixed_precision_free_run():
         # (full-step exponential against the symmetric checkerboard), O(1) when
         # many flips are accepted between recomputes; it must stay finite
         assert np.isfinite(xmax) and xmax < 10.0
+
+
+@pytest.mark.parametrize("therm", [(2, 2, 10), (4, 2, 20), (2, 4, 5)])
+def test_thermalisation_size_transfer(therm):
+    """conf_set at the thermalisation size then conf_transfer (src/fields.f90:429-507)
+    onto L = 4, Ltrot = 20: field, lambda and seed as the oracle's, then one sweep"""
+    import z2_gauge_zfy_amd as z
+    LxT, LyT, TT = therm
+    specs = [z.ChainSpec(h=0.5, Beta=2.0, seed=2024 + 2 * 7919 * k, Q=(1.0 if k else -1.0)) for k in range(2)]
+    e = z.Engine(4, 4, 20, 5, specs)
+    e.init_fields_therm(LxT, LyT, TT)
+    for k, s in enumerate(specs):
+        p = O.make_params(RT=1.0, J=1.0, h=0.5, mu=0.0, Q=s.Q, Nlx=4, Nly=4, Ltrot=20, Beta=2.0, Nwrap=5,
+                          NlxTherm=LxT, NlyTherm=LyT, LtrotTherm=TT)
+        c = O.Chain(O.Model(p), s.seed)
+        assert np.array_equal(e.sigma(k), c.sigma()), k
+        assert np.array_equal(e.lam(k), c.lam()), k
+        assert e.seed(k) == c.seed, k
+    e.pre()
+    e.sweep(True)

@@ -105,6 +105,9 @@ _SIGS = {
     "z2_get_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "z2_get_lambda": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "z2_set_lambda": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "z2_set_sigma_therm": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int32]),
+    "z2_init_fields_therm": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "z2_set_is_beta": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "z2_set_stab_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_get_is_beta": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]),

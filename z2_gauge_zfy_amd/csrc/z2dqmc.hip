@@ -763,6 +763,61 @@ extern "C" int z2_set_lambda(z2_ctx *c, int32_t chain, const double *lambda) {
   return Z2_OK;
 }
 
+// conf_transfer, src/fields.f90:429-507: the field of a thermalisation-size lattice
+// (LxT x LyT sites, TT slices) tiled periodically onto the run's lattice (bond of
+// site (x, y) in direction d <- bond of site ((x-1) % LxT + 1, (y-1) % LyT + 1) in
+// direction d; slice nt <- slice nt - TT beyond TT), lambda likewise by site, then
+// the sector fix (lambda_enforce_sector, :580-603)
+static int conf_transfer_host(const z2_ctx *c, int LxT, int LyT, int TT, double Q, const double *sig_t,
+                              const double *lam_t, std::vector<double> &sig, double *lam) {
+  if (LxT < 1 || LyT < 1 || TT < 1 || TT > c->T)
+    return set_err(Z2_EINVAL, "thermalisation size %d x %d x %d does not tile %d x %d x %d", LxT, LyT, TT, c->Lx,
+                   c->Ly, c->T);
+  const int N2 = c->N2, N2T = 2 * LxT * LyT;
+  sig.assign((size_t)N2 * c->T, 0.0);
+  for (int nt = 0; nt < TT; ++nt)
+    for (int b = 0; b < N2; ++b) {
+      const int site = b / 2, d = b % 2, x = site % c->Lx, y = site / c->Lx;
+      const int bt = 2 * ((y % LyT) * LxT + (x % LxT)) + d;
+      sig[(size_t)nt * N2 + b] = sig_t[(size_t)nt * N2T + bt];
+    }
+  for (int nt = TT; nt < c->T; ++nt)
+    for (int b = 0; b < N2; ++b) sig[(size_t)nt * N2 + b] = sig[(size_t)(nt - TT) * N2 + b];
+  double prod = 1.0;
+  for (int i = 0; i < c->N; ++i) {
+    const int x = i % c->Lx, y = i / c->Lx;
+    lam[i] = lam_t[(y % LyT) * LxT + (x % LxT)] >= 0.0 ? 1.0 : -1.0;
+    prod = prod * lam[i];
+  }
+  if (prod * (Q >= 0.0 ? 1.0 : -1.0) < 0.0) lam[c->N - 1] = -lam[c->N - 1];
+  return Z2_OK;
+}
+
+extern "C" int z2_set_sigma_therm(z2_ctx *c, int32_t chain, int32_t NlxTherm, int32_t NlyTherm, int32_t LtrotTherm,
+                                  const double *sigma_therm, const double *lambda_therm, int32_t seed) {
+  if (!c || !sigma_therm || !lambda_therm || chain < 0 || chain >= c->nch_all)
+    return set_err(Z2_EINVAL, "bad argument");
+  std::vector<double> sig, lam(c->N);
+  CHK(conf_transfer_host(c, NlxTherm, NlyTherm, LtrotTherm, c->desc[c->phys[chain]].Q, sigma_therm, lambda_therm,
+                         sig, lam.data()));
+  CHK(z2_set_sigma(c, chain, sig.data(), seed));
+  return z2_set_lambda(c, chain, lam.data());
+}
+
+extern "C" int z2_init_fields_therm(z2_ctx *c, int32_t NlxTherm, int32_t NlyTherm, int32_t LtrotTherm) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  if (NlxTherm < 1 || NlyTherm < 1 || LtrotTherm < 1) return set_err(Z2_EINVAL, "bad thermalisation size");
+  std::vector<double> sig_t, lam_t((size_t)NlxTherm * NlyTherm);
+  for (int k = 0; k < c->nch_all; ++k) {
+    const z2_chain_desc &d = c->desc[c->phys[k]];
+    if (d.absolute < 1 || d.absolute > 5) return set_err(Z2_EINVAL, "absolute must be 1..5");
+    int32_t s = d.seed;
+    conf_set_host(NlxTherm, NlyTherm, LtrotTherm, d.absolute, d.Q, &s, sig_t, lam_t.data());
+    CHK(z2_set_sigma_therm(c, k, NlxTherm, NlyTherm, LtrotTherm, sig_t.data(), lam_t.data(), s));
+  }
+  return Z2_OK;
+}
+
 extern "C" int z2_get_seed(z2_ctx *c, int32_t chain, int32_t *seed) {
   if (!c || !seed || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
   chain = c->phys[chain];  // storage slot

@@ -66,6 +66,10 @@ class Engine:
     def init_fields(self):
         check(lib().z2_init_fields(self._ptr))
 
+    def init_fields_therm(self, NlxTherm: int, NlyTherm: int, LtrotTherm: int):
+        """fresh start at the thermalisation size, tiled onto the lattice (conf_transfer)"""
+        check(lib().z2_init_fields_therm(self._ptr, NlxTherm, NlyTherm, LtrotTherm))
+
     def set_sigma(self, chain: int, sigma: np.ndarray, seed: int):
         s = np.ascontiguousarray(sigma, dtype=np.float64).reshape(-1)
         assert s.size == 2 * self.N * self.T
