@@ -832,7 +832,7 @@ __global__ void __launch_bounds__(512) k_qr_order(QrbArgs a) {
 
 // Panel p of one matrix: factored from the m x w block at Wp (column stride ldW,
 // global or LDS) into L.Pn; writes its columns of A (normalised R rows of the
-// diagonal block, signs, reflectors), tau, D and, if columns follow, its T into Tws.
+// diagonal block, signs, reflectors), tau, D and its T into Tws.
 template <int NB, int RPL>
 __device__ void qrb_panel_step(const QrbArgs &a, QrbLds &L, double *Tws, int chain, int p, const double *Wp,
                                int ldW) {
@@ -844,11 +844,11 @@ __device__ void qrb_panel_step(const QrbArgs &a, QrbLds &L, double *Tws, int cha
   double *D = a.D + (size_t)chain * N;
   const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
   qrb_panel_reg<NB, RPL>(L, Wp, ldW, nullptr, m, w, N - 1 - j0);
-  if (j0 + w < N) {
-    qrb_build_t<NB>(L, m, w);
-    double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
-    for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) T[e] = L.Tm[(e % NB) + (size_t)(e / NB) * L.lw];
-  }
+  // T of every panel, the last one too: the explicit Q and the Q^T application
+  // that follow the factorization use them as they are
+  qrb_build_t<NB>(L, m, w);
+  double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) T[e] = L.Tm[(e % NB) + (size_t)(e / NB) * L.lw];
   for (int c = wave; c < w; c += nw)
 #pragma unroll
     for (int t = 0; t < RPL; ++t) {

@@ -112,6 +112,7 @@ struct z2_ctx {
   int *ipvt = nullptr;
   double *qrw = nullptr, *qrw2 = nullptr;  // workspaces of the blocked QR (N x N, 2N x 2N)
   double *ortw = nullptr;  // panel T factors of the explicit Q (ceil(N/16) x 32 x 32 per chain)
+  const double *ortw_src = nullptr;  // the matrix whose split QR wrote the T factors now in ortw
   size_t ortw_per = 0;
   int qr_mode = Z2_QR_BLOCKED;
   int stab_mode = Z2_STAB_REFERENCE;
@@ -1274,15 +1275,17 @@ static int launch_qrb(int orgqr, int nb, int nmat, size_t sh, hipStream_t st, co
 // explicit Q over many workgroups per matrix (panel T factors, then one launch per panel)
 template <int NB, int RPL>
 static int launch_orgqr_split_t(int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q, double *Tws,
-                                int apply = 0) {
+                                int apply = 0, int have_t = 0) {
   const int npan = (n + NB - 1) / NB;
   if (sh > 65536) {
     HIPC(hipFuncSetAttribute((const void *)k_orgqr_t<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     HIPC(hipFuncSetAttribute((const void *)k_orgqr_tiles<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     HIPC(hipFuncSetAttribute((const void *)k_qt_apply<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   }
-  hipLaunchKernelGGL((k_orgqr_t<NB, RPL>), dim3(npan, nmat), dim3(512), sh, st, q, Tws);
-  LAUNCH_CHECK();
+  if (!have_t) {  // the factorization's own T factors are not at hand: rebuild them
+    hipLaunchKernelGGL((k_orgqr_t<NB, RPL>), dim3(npan, nmat), dim3(512), sh, st, q, Tws);
+    LAUNCH_CHECK();
+  }
   if (apply)
     hipLaunchKernelGGL((k_qt_apply<NB, RPL>), dim3((n + tc - 1) / tc, nmat), dim3(512), sh, st, q,
                        (const double *)Tws);
@@ -1295,16 +1298,16 @@ static int launch_orgqr_split_t(int n, int tc, int nmat, size_t sh, hipStream_t 
 
 template <int NB>
 static int launch_orgqr_split_nb(int rpl, int n, int tc, int nmat, size_t sh, hipStream_t st, const QrbArgs &q,
-                                 double *Tws, int apply = 0) {
+                                 double *Tws, int apply = 0, int have_t = 0) {
   switch (rpl) {
-    case 1: return launch_orgqr_split_t<NB, 1>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 2: return launch_orgqr_split_t<NB, 2>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 3: return launch_orgqr_split_t<NB, 3>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 4: return launch_orgqr_split_t<NB, 4>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 5: return launch_orgqr_split_t<NB, 5>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 6: return launch_orgqr_split_t<NB, 6>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 7: return launch_orgqr_split_t<NB, 7>(n, tc, nmat, sh, st, q, Tws, apply);
-    case 8: return launch_orgqr_split_t<NB, 8>(n, tc, nmat, sh, st, q, Tws, apply);
+    case 1: return launch_orgqr_split_t<NB, 1>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 2: return launch_orgqr_split_t<NB, 2>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 3: return launch_orgqr_split_t<NB, 3>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 4: return launch_orgqr_split_t<NB, 4>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 5: return launch_orgqr_split_t<NB, 5>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 6: return launch_orgqr_split_t<NB, 6>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 7: return launch_orgqr_split_t<NB, 7>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
+    case 8: return launch_orgqr_split_t<NB, 8>(n, tc, nmat, sh, st, q, Tws, apply, have_t);
   }
   return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
 }
@@ -1376,6 +1379,7 @@ static int qrp_n(z2_ctx *c, double *A, double *D, double *tau, int *ipvt, int n,
       const size_t sh = qrb_lds_bytes(n, nb, tc);
       if (nb == 32) CHK(launch_qr_split_nb<32>(c, rpl, sh, q));
       else CHK(launch_qr_split_nb<16>(c, rpl, sh, q));
+      c->ortw_src = A;  // every panel's T factor of this factorization is now in ortw
     } else {
       CHK(launch_qrb(0, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
     }
@@ -1412,8 +1416,10 @@ static int orgqr_n(z2_ctx *c, const double *A, const double *tau, double *Qo, in
     if (n == c->N && c->ortw) {
       const int rpl = (n + 63) / 64;
       const size_t sh = qrb_lds_bytes(n, nb, tc);
-      if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw));
-      else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw));
+      const int have_t = c->ortw_src == A;
+      if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 0, have_t));
+      else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 0, have_t));
+      c->ortw_src = A;  // rebuilt or kept: ortw holds A's T factors
     } else {
       CHK(launch_qrb(1, nb, c->nch, qrb_lds_bytes(n, nb, tc), c->stream, q));
     }
@@ -1450,8 +1456,10 @@ static int apply_qt(z2_ctx *c, const double *A, const double *B, double *X, int 
     CHK(prof_begin(c, Z2_K_QT_APPLY));
     const int rpl = (n + 63) / 64;
     const size_t sh = qrb_lds_bytes(n, nb, tc);
-    if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1));
-    else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1));
+    const int have_t = c->ortw_src == A;
+    if (nb == 32) CHK(launch_orgqr_split_nb<32>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1, have_t));
+    else CHK(launch_orgqr_split_nb<16>(rpl, n, tc, c->nch, sh, c->stream, q, c->ortw, 1, have_t));
+    c->ortw_src = A;
     // Q^T applied to an n x n right-hand side: sum_j 4 (n - j) n = 2 n^3 flops
     CHK(prof_end(c, Z2_K_QT_APPLY, 2.0 * n * n * (double)n * c->nch));
     return Z2_OK;
