@@ -1,4 +1,4 @@
-"""Pins of three oracle functions that the reference's own fixtures do not reach,
+"""Pins of four oracle functions that the reference's own fixtures do not reach,
 each against what the mathematics fixes:
 
 * conf_set with absolute = 1 (src/fields.f90:322-333) and lambda_random_fill +
@@ -12,7 +12,9 @@ each against what the mathematics fixes:
   where G(0,0) = G(t,t) = G(t,0) = G and G(0,t) = -(I - G)
   (src/process_matrix.f90:88-96): sums over displacements against Obs_equal_calc's
   own sums on the same G (src/obser_equal.f90:73-149, pinned by the reference's
-  test_run series) and against trace identities evaluated by matrix algebra.
+  test_run series) and against trace identities evaluated by matrix algebra;
+* conf_transfer (src/fields.f90:429-507): the run-size field as the periodic
+  tiling of the thermalisation-size field, built here by index arithmetic.
 """
 import itertools
 
@@ -163,3 +165,37 @@ def test_obs_tau_first_slice_identities():
     tsum = (-np.trace(X @ S @ Y @ S) - np.trace(S.T @ X @ S.T @ Y) + np.trace(X @ S.T @ Y @ S)
             + np.trace(S.T @ X @ S @ Y))
     assert o["curxx"][:, 0].sum() == pytest.approx(-4.0 * RT * RT * tsum, rel=1e-10, abs=1e-10)
+
+
+# --------------------------------------------------- conf_transfer (a3)
+@pytest.mark.parametrize("LxT, LyT, TT, L, T, Q", [(2, 2, 10, 4, 20, 1.0), (4, 2, 5, 4, 20, -1.0),
+                                                  (2, 4, 20, 4, 20, 1.0), (3, 3, 4, 6, 12, -1.0)])
+def test_conf_transfer_is_the_periodic_tiling(LxT, LyT, TT, L, T, Q):
+    """src/fields.f90:429-507: every bond of the run-size lattice takes the value of
+    the bond of the same direction at the site (x mod LxT, y mod LyT) of the
+    thermalisation-size field, slice nt takes slice (nt - 1) mod TT + 1, lambda is
+    tiled the same way and then put into the sector of Q (last site flipped when the
+    product has the wrong sign); the seed is the one the thermalisation-size conf_set
+    leaves (conf_set's draws are pinned above)"""
+    seed = 97531
+    pT = O.make_params(Nlx=LxT, Nly=LyT, Ltrot=TT, Beta=1.0, Nwrap=TT, Q=Q, absolute=1)
+    cT = O.Chain(O.Model(pT), seed)
+    sT, lT = cT.sigma(), cT.lam()
+    p = O.make_params(Nlx=L, Nly=L, Ltrot=T, Beta=1.0, Nwrap=T, Q=Q, absolute=1,
+                      NlxTherm=LxT, NlyTherm=LyT, LtrotTherm=TT)
+    c = O.Chain(O.Model(p), seed)
+    N = L * L
+    sig = np.empty((T, 2 * N))
+    lam = np.empty(N)
+    for y in range(L):
+        for x in range(L):
+            s, st = y * L + x, (y % LyT) * LxT + (x % LxT)
+            for d in range(2):  # 0: +x bond 2s-1, 1: +y bond 2s (1-based)
+                for nt in range(T):
+                    sig[nt, 2 * s + d] = sT[nt % TT, 2 * st + d]
+            lam[s] = lT[st]
+    if np.prod(lam) * (1.0 if Q >= 0 else -1.0) < 0:
+        lam[-1] = -lam[-1]
+    assert np.array_equal(c.sigma(), sig)
+    assert np.array_equal(c.lam(), lam)
+    assert c.seed == cT.seed
