@@ -201,6 +201,15 @@ int z2_get_counters(z2_ctx *ctx, int32_t chain, uint64_t out[2]);
  * accepts (default min(16, kmax(N)), kmax set by the LDS size); the decisions
  * do not depend on k, G only through rounding */
 int z2_set_delay(z2_ctx *ctx, int32_t k);
+/* local-update kernel variant (no counterpart in the reference): in the mixed
+ * mode at N = 256, Z2_FLIP_REGISTER (default) keeps the chain's FP32 G in the
+ * registers of its workgroup for the whole slice and applies every pair of
+ * accepted updates as one rank-4 matrix-core product there (no delay depth);
+ * Z2_FLIP_DELAYED is the delayed-update kernel used everywhere else.  The
+ * decisions are the serial ones in both; G differs by rounding. */
+#define Z2_FLIP_DELAYED 0
+#define Z2_FLIP_REGISTER 1
+int z2_set_flip_variant(z2_ctx *ctx, int32_t variant);
 /* G recompute of the stabilization (no counterpart in the reference):
  * Z2_STAB_REFERENCE (default) is stab_green (src/stabilization.f90:122-200):
  * M = (UL UR)^T + DR (VR VL) DL, column-pivoted QR, G = UL^T P V^-1 D^-1 Q^T UR^T,

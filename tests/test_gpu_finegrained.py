@@ -57,7 +57,7 @@ def fine_sweep(e, T, w, toggle):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("L, T, w", [(4, 20, 5), (8, 24, 6)])
+@pytest.mark.parametrize("L, T, w", [(4, 20, 5), (8, 24, 6), (16, 20, 10)])
 def test_fine_grained_sweep_matches_batched(precision, L, T, w):
     import z2_gauge_zfy_amd as z
     prec = z.Z2_PREC_FP64 if precision == "fp64" else z.Z2_PREC_MIXED

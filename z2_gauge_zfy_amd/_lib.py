@@ -21,6 +21,8 @@ Z2_STAB_REFERENCE = 0
 Z2_STAB_DSPLIT = 1
 Z2_EXEC_EAGER = 0
 Z2_EXEC_GRAPH = 1
+Z2_FLIP_DELAYED = 0
+Z2_FLIP_REGISTER = 1
 
 # kernel classes for z2_profile_kernel
 KERNEL_IDS = {"qrp": 1, "flip": 2, "kin_rows": 3, "kin_cols": 4, "gemm": 5, "orgqr": 6, "trsm": 7,
@@ -118,6 +120,7 @@ _SIGS = {
     "z2_get_therm_counters": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "z2_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "z2_set_delay": (C.c_int, [C.c_void_p, C.c_int32]),
+    "z2_set_flip_variant": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_set_qr_mode": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),

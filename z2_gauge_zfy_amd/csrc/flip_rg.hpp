@@ -1,0 +1,385 @@
+// Local link-flip sweep of one time slice with G held in registers (FP32 G,
+// N = 256): the reference's LocalK_metro (src/localK.f90:45-173) in the order of
+// LocalK_prop_L/R (:347-398), as k_flip, for the mixed mode at the size where
+// the FP32 G of one chain (256 KiB) fits in the register file of one compute
+// unit (512 KiB).
+//
+// One workgroup (8 wavefronts) per chain, thread k owns proposal k.  G is
+// loaded once into the wavefronts' registers in the matrix cores' accumulator
+// layout: wavefront w holds columns 32w .. 32w+31 as 16 x 2 tiles of 16 x 16,
+// lane (li, lk) item q of tile (tr, tc) is G(16 tr + 4 lk + q, 32 w + 16 tc + li).
+// Per accepted proposal:
+//   (E) every undecided proposal folds the last accepted update into its 2x2
+//       block (registers) and is evaluated with its cached boson ratio; a
+//       wavefront min and one LDS atomic give the first accepted one (the
+//       serial decision, as in k_flip);
+//   (X) the lanes holding columns p1, p2 and rows p1, p2 of G copy them to LDS;
+//   (U) 2N threads build the update vectors u = G[:,P] Prod^{-1} and
+//       v = Delta (I - G)[P,:] in FP64, minus the updates not yet in the
+//       registers (at most two);
+// with a barrier after each phase.
+// Accepted updates are applied to the registers in pairs, one rank-4 product
+// per tile on the FP32 matrix cores, issued after the extraction that follows
+// their second member; a lone last update is applied at the end.  Nothing is
+// left pending for the kinetic kernels: G is written back whole, with the
+// exp(dtau mu) factors applied on load (left pass) or on store (right pass) as
+// k_flip applies them.
+#pragma once
+#include "flip_kernels.hpp"
+
+typedef float rg_f4 __attribute__((ext_vector_type(4)));
+
+// LDS bytes of k_flip_rg for N sites
+__host__ __device__ inline size_t flip_rg_lds_bytes(int N) {
+  const size_t N2 = 2 * (size_t)N;
+  size_t b = sizeof(double) * (4 * 4 * (size_t)N + 4 * N2 + 16);  // 4 update slots, 2x2 per bond, boson
+  b += sizeof(float) * (2 + 8) * (size_t)N;  // columns p1, p2; rows p1, p2 as float4 per column
+  b += sizeof(short) * 8 * N2;               // bond sites (2) and plaquette partner bonds (6) per bond
+  b += 2 * N2;                               // current field, sum of the neighbouring slices' fields
+  return (b + 15) & ~(size_t)15;
+}
+
+// the rank-4 (both = true) or rank-2 update of the register tiles:
+// G(r, c) -= sum_k u_k(r) v_k(c) over the vectors of slots sa (and sb)
+template <int TR>
+__device__ __forceinline__ void rg_update(rg_f4 (&g)[TR][2], const double *US, int N, int wave, int li,
+                                          int lk, int sa, int sb, bool both) {
+  const int cmp = lk & 1;
+  const double *U = US + (size_t)4 * N * (lk < 2 ? sa : sb);
+  const bool on = lk < 2 || both;
+  float bv[2];
+#pragma unroll
+  for (int tc = 0; tc < 2; ++tc) {
+    const float v = (float)(-U[2 * N + 2 * (32 * wave + 16 * tc + li) + cmp]);
+    bv[tc] = on ? v : 0.f;  // -v_k(col)
+  }
+#pragma unroll
+  for (int tr = 0; tr < TR; ++tr) {
+    const float t = (float)U[2 * (16 * tr + li) + cmp];
+    const float av = on ? t : 0.f;  // u_k(row)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc) g[tr][tc] = mfma_16x16x4<float>(av, bv[tc], g[tr][tc]);
+  }
+}
+
+// boson ratio of flipping the field s_old of a bond with partner bonds bb and
+// neighbouring-slice sum ss (src/fields.f90:66-117), from the current field sg
+__device__ __forceinline__ double rg_boson(const int8_t *sg, const double *bos, const int (&bb)[6], int s_old, int ss) {
+  const int s_new = -s_old;
+  const int tn1 = s_new * (int)sg[bb[0]] * (int)sg[bb[1]] * (int)sg[bb[2]];
+  const int tn2 = s_new * (int)sg[bb[3]] * (int)sg[bb[4]] * (int)sg[bb[5]];
+  const int dS = 2 * (tn1 + tn2);
+  const int dT = 2 * s_new * ss;
+  return bos[(dS / 4 + 1) * 3 + (dT / 4 + 1)];
+}
+
+// sum over c of T_c(i) V_c(j) for the update vectors of one slot
+__device__ __forceinline__ double rg_dot2(const double *T, int i, const double *V, int j) {
+  return T[2 * i] * V[2 * j] + T[2 * i + 1] * V[2 * j + 1];
+}
+
+template <int NC>
+__global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
+  static_assert(NC == 256, "register-resident G: 8 wavefronts x 32 columns");
+  constexpr int N = NC, N2 = 2 * NC, TR = NC / 16, nth = 512;
+  const int chain = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const ChainConst *cp = a.cc + chain;
+  const double ep = cp->ep, em = cp->em;
+  float *G = a.G + (size_t)chain * N * N;
+  int8_t *sigc = a.sig + (size_t)chain * a.T * N2;
+  int8_t *sig_t = sigc + (size_t)(a.nt - 1) * N2;
+  const int tp = (a.nt - 2 + a.T) % a.T, tn = a.nt % a.T;  // 0-based neighbour slices
+  const int8_t *sig_p = sigc + (size_t)tp * N2, *sig_n = sigc + (size_t)tn * N2;
+
+  extern __shared__ unsigned char smem_raw[];
+  double *US = reinterpret_cast<double *>(smem_raw);   // 4 x 4N: update slots, u_c(i) at 2i + c,
+                                                       //         v_c(i) at 2N + 2i + c
+  double *o4 = US + (size_t)16 * N;                    // 2N x 4: 2x2 block of an accepted candidate
+  double *bos = o4 + (size_t)4 * N2;                   // 9 (16): boson ratio table
+  float *col = reinterpret_cast<float *>(bos + 16);    // 2 x N : G(:,p1), G(:,p2)
+  float *row4 = col + 2 * N;                           // 2 x N x 4: G(p,:) as the float4 of rows
+                                                       //   4(p/4) .. 4(p/4)+3 per column
+  short *bp = reinterpret_cast<short *>(row4 + 8 * N); // 2N x 2: bond sites
+  short *bbs = bp + 2 * N2;                            // 2N x 6: plaquette partner bonds
+  int8_t *sg = reinterpret_cast<int8_t *>(bbs + 6 * N2);  // 2N : current slice field
+  int8_t *sgs = sg + N2;                                  // 2N : sig(nt-1) + sig(nt+1)
+  __shared__ int s_kmin[2];
+  __shared__ uint32_t s_dec[64];
+
+  const double Dp0 = cp->delta[0][0], Dp1 = cp->delta[0][1], Dp2 = cp->delta[0][2], Dp3 = cp->delta[0][3];
+  const double Dm0 = cp->delta[1][0], Dm1 = cp->delta[1][1], Dm2 = cp->delta[1][2], Dm3 = cp->delta[1][3];
+  const int mu = a.mu_on_read;
+
+  // G into the registers (with the mu factors of a left pass, as flip_read)
+  rg_f4 g[TR][2];
+#pragma unroll
+  for (int tr = 0; tr < TR; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc) {
+      rg_f4 x = *reinterpret_cast<const rg_f4 *>(G + 16 * tr + 4 * lk + (size_t)(32 * wave + 16 * tc + li) * N);
+      if (mu) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          x[q] = x[q] * (float)ep;
+          x[q] = x[q] * (float)em;
+        }
+      }
+      g[tr][tc] = x;
+    }
+  const uint32_t seed0 = (uint32_t)a.seed[chain] & 0x7fffffffu;
+  for (int b = tid; b < N2; b += nth) {
+    bp[2 * b] = (short)a.lat.bond_p[2 * b];
+    bp[2 * b + 1] = (short)a.lat.bond_p[2 * b + 1];
+    for (int q = 0; q < 6; ++q) bbs[6 * b + q] = (short)a.lat.bond_bonds[6 * b + q];
+    sg[b] = sig_t[b];
+    sgs[b] = (int8_t)((int)sig_p[b] + (int)sig_n[b]);
+  }
+  if (tid < 9) bos[tid] = cp->boson[tid];
+  for (int q = tid; q < 64; q += nth) s_dec[q] = 0u;
+  if (tid == 0) {
+    s_kmin[0] = 2 * N2 + 2;
+    s_kmin[1] = 2 * N2 + 2;
+  }
+  const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
+  const int gb = a.gbond ? (a.gmask[(size_t)chain * a.T + (a.nt - 1)] ? a.gbond[chain] : -1) : -1;
+  __syncthreads();
+  // this thread's proposal k = tid (2N = 512 threads): bond, sites, partner bonds,
+  // field, random number (draw k+1 of the slice), 2x2 block, boson ratio
+  const int k = tid;
+  const int rb_ = a.descending ? (N2 - 1 - k) : k;
+  const int rp1 = bp[2 * rb_], rp2 = bp[2 * rb_ + 1];
+  int rbb[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) rbb[q] = bbs[6 * rb_ + q];
+  const int rso = sg[rb_], rss = sgs[rb_];
+  const double rrn = (double)(uint32_t)(((uint64_t)seed0 * (uint64_t)a.apow[k + 1]) & 0x7fffffffull) / 2147483648.0;
+  double r11 = (double)flip_read<float>(G, rp1, rp1, N, mu, ep, em);
+  double r22 = (double)flip_read<float>(G, rp2, rp2, N, mu, ep, em);
+  double r12 = (double)flip_read<float>(G, rp1, rp2, N, mu, ep, em);
+  double r21 = (double)flip_read<float>(G, rp2, rp1, N, mu, ep, em);
+  double rbo = rg_boson(sg, bos, rbb, rso, rss);
+  const double D0 = rso > 0 ? Dp0 : Dm0, D1 = rso > 0 ? Dp1 : Dm1;
+  const double D2 = rso > 0 ? Dp2 : Dm2, D3 = rso > 0 ? Dp3 : Dm3;
+
+  int start = 0, round = 0, tot = 0, last = -1, lastb = -1;
+  int pq0 = -1, pq1 = -1;  // slots of the accepted updates not yet in the registers, in order
+  while (true) {
+    const int slot = round & 1;
+    // (E) fold the last accepted update into this thread's undecided proposal, evaluate it
+    int cand = 2 * N2 + 2;
+    if (k >= start) {
+      if (last >= 0) {
+        const double *TA = US + (size_t)4 * N * last, *VA = TA + 2 * N;
+        const double d12 = rg_dot2(TA, rp1, VA, rp2), d21 = rg_dot2(TA, rp2, VA, rp1);
+        const double e1 = rg_dot2(TA, rp1, VA, rp1), e2 = rg_dot2(TA, rp2, VA, rp2);
+        r12 = r12 - d12;
+        r21 = r21 - d21;
+        r11 = r11 - e1;
+        r22 = r22 - e2;
+        // the boson ratio changes only when a partner bond flipped
+        if (rbb[0] == lastb || rbb[1] == lastb || rbb[2] == lastb || rbb[3] == lastb || rbb[4] == lastb ||
+            rbb[5] == lastb)
+          rbo = rg_boson(sg, bos, rbb, rso, rss);
+      }
+      const double A11 = 1.0 - r11, A21 = 0.0 - r21;
+      const double A12 = 0.0 - r12, A22 = 1.0 - r22;
+      const double P11 = 1.0 + (D0 * A11 + D2 * A21);
+      const double P21 = 0.0 + (D1 * A11 + D3 * A21);
+      const double P12 = 0.0 + (D0 * A12 + D2 * A22);
+      const double P22 = 1.0 + (D1 * A12 + D3 * A22);
+      const double det = P11 * P22 - P12 * P21;
+      const double ratio = (det * det) * rbo;
+      const bool acc = a.gbond ? rb_ == gb
+                               : (frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rrn);
+      if (acc) {
+        cand = 2 * k + (rso > 0 ? 1 : 0);
+        double *o = o4 + 4 * rb_;
+        o[0] = r11;
+        o[1] = r21;
+        o[2] = r12;
+        o[3] = r22;
+      }
+    }
+    cand = wave_min_int(cand);
+    if (lane == 0 && cand < 2 * N2) atomicMin(&s_kmin[slot], cand);
+    if (tid == 0) s_kmin[slot ^ 1] = 2 * N2 + 2;
+    __syncthreads();
+    const int packed = s_kmin[slot];
+    if (packed >= 2 * N2) break;
+    const int kmin = packed >> 1, s_old = (packed & 1) ? 1 : -1;
+    const int b = a.descending ? (N2 - 1 - kmin) : kmin;
+    const int p1 = __builtin_amdgcn_readfirstlane(bp[2 * b]), p2 = __builtin_amdgcn_readfirstlane(bp[2 * b + 1]);
+    // (X) columns p1, p2 and rows p1, p2 of the register G into LDS
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int P = j ? p2 : p1;
+      if (wave == (P >> 5) && li == (P & 15)) {
+        if ((P >> 4) & 1) {
+#pragma unroll
+          for (int tr = 0; tr < TR; ++tr) *reinterpret_cast<rg_f4 *>(col + j * N + 16 * tr + 4 * lk) = g[tr][1];
+        } else {
+#pragma unroll
+          for (int tr = 0; tr < TR; ++tr) *reinterpret_cast<rg_f4 *>(col + j * N + 16 * tr + 4 * lk) = g[tr][0];
+        }
+      }
+      // row P: item P & 3 of the lanes with lk = (P >> 2) & 3 in tile row P >> 4; the whole
+      // float4 is stored and the item picked by address when it is read
+      rg_f4 x0 = g[0][0], x1 = g[0][1];
+      switch (P >> 4) {
+#define RG_ROW_CASE(t) \
+  case t:              \
+    x0 = g[t][0];      \
+    x1 = g[t][1];      \
+    break;
+        RG_ROW_CASE(0) RG_ROW_CASE(1) RG_ROW_CASE(2) RG_ROW_CASE(3) RG_ROW_CASE(4) RG_ROW_CASE(5)
+        RG_ROW_CASE(6) RG_ROW_CASE(7) RG_ROW_CASE(8) RG_ROW_CASE(9) RG_ROW_CASE(10) RG_ROW_CASE(11)
+        RG_ROW_CASE(12) RG_ROW_CASE(13) RG_ROW_CASE(14) RG_ROW_CASE(15)
+#undef RG_ROW_CASE
+        default: break;
+      }
+      if (lk == ((P >> 2) & 3)) {
+        *reinterpret_cast<rg_f4 *>(row4 + 4 * (j * N + 32 * wave + li)) = x0;
+        *reinterpret_cast<rg_f4 *>(row4 + 4 * (j * N + 32 * wave + 16 + li)) = x1;
+      }
+    }
+    // the accepted proposal's 2x2 block as its thread evaluated it, and Prod^{-1}
+    const double D11 = s_old > 0 ? Dp0 : Dm0, D21 = s_old > 0 ? Dp1 : Dm1;
+    const double D12 = s_old > 0 ? Dp2 : Dm2, D22 = s_old > 0 ? Dp3 : Dm3;
+    double Pi11, Pi21, Pi12, Pi22;
+    {
+      const double *o = o4 + 4 * b;
+      const double A11 = 1.0 - o[0], A21 = 0.0 - o[1];
+      const double A12 = 0.0 - o[2], A22 = 1.0 - o[3];
+      const double P11 = 1.0 + (D11 * A11 + D12 * A21);
+      const double P21 = 0.0 + (D21 * A11 + D22 * A21);
+      const double P12 = 0.0 + (D11 * A12 + D12 * A22);
+      const double P22 = 1.0 + (D21 * A12 + D22 * A22);
+      const double det = P11 * P22 - P12 * P21;
+      const double rdet = 1.0 / det;
+      Pi11 = P22 * rdet;
+      Pi21 = (-P21) * rdet;
+      Pi12 = (-P12) * rdet;
+      Pi22 = P11 * rdet;
+      if (a.gbond && tid == 0) {
+        double r = fabs(det * det);
+        a.glr[2 * chain] = a.glr[2 * chain] + log(r > 1.0e-300 ? r : 1.0e-300);
+      }
+    }
+    if (tid == 0) {
+      sg[b] = (int8_t)(-s_old);
+      s_dec[kmin >> 5] |= 1u << (kmin & 31);
+      tot = tot + 1;
+    }
+    __syncthreads();
+    // (U) this accept's update vectors in slot ns, from the extraction minus the
+    // pending updates pq0, pq1 (FP64)
+    const int q1 = p1 & 3, q2 = p2 & 3;
+    const double *T0 = pq0 >= 0 ? US + (size_t)4 * N * pq0 : nullptr, *V0 = pq0 >= 0 ? T0 + 2 * N : nullptr;
+    const double *T1 = pq1 >= 0 ? US + (size_t)4 * N * pq1 : nullptr, *V1 = pq1 >= 0 ? T1 + 2 * N : nullptr;
+    // G(i, p1), G(i, p2) and G(p1, i), G(p2, i) of the current G at site i
+    auto colv = [&](int i, double &c1, double &c2) {
+      c1 = (double)col[i];
+      c2 = (double)col[N + i];
+      if (pq0 >= 0) {
+        c1 = c1 - rg_dot2(T0, i, V0, p1);
+        c2 = c2 - rg_dot2(T0, i, V0, p2);
+      }
+      if (pq1 >= 0) {
+        c1 = c1 - rg_dot2(T1, i, V1, p1);
+        c2 = c2 - rg_dot2(T1, i, V1, p2);
+      }
+    };
+    auto rowv = [&](int i, double &r1, double &r2) {
+      r1 = (double)row4[4 * i + q1];
+      r2 = (double)row4[4 * (N + i) + q2];
+      if (pq0 >= 0) {
+        r1 = r1 - rg_dot2(T0, p1, V0, i);
+        r2 = r2 - rg_dot2(T0, p2, V0, i);
+      }
+      if (pq1 >= 0) {
+        r1 = r1 - rg_dot2(T1, p1, V1, i);
+        r2 = r2 - rg_dot2(T1, p2, V1, i);
+      }
+    };
+    auto u_at = [&](int i, double &u0, double &u1) {
+      double c1, c2;
+      colv(i, c1, c2);
+      u0 = c1 * Pi11 + c2 * Pi21;
+      u1 = c1 * Pi12 + c2 * Pi22;
+    };
+    auto v_at = [&](int i, double &v0, double &v1) {
+      double r1, r2;
+      rowv(i, r1, r2);
+      v0 = -(D11 * r1) - D12 * r2;
+      v1 = -(D21 * r1) - D22 * r2;
+      if (i == p1) {
+        v0 = v0 + D11;
+        v1 = v1 + D21;
+      }
+      if (i == p2) {
+        v0 = v0 + D12;
+        v1 = v1 + D22;
+      }
+    };
+    const int ns = round & 3;
+    double *TN = US + (size_t)4 * N * ns;
+    if (tid < N) {
+      double u0, u1;
+      u_at(tid, u0, u1);
+      TN[2 * tid] = u0;
+      TN[2 * tid + 1] = u1;
+    } else {
+      double v0, v1;
+      v_at(tid - N, v0, v1);
+      TN[2 * N + 2 * (tid - N)] = v0;
+      TN[2 * N + 2 * (tid - N) + 1] = v1;
+    }
+    // the two pending updates into the registers (the extraction above has read them)
+    if (pq1 >= 0) rg_update<TR>(g, US, N, wave, li, lk, pq0, pq1, true);
+    __syncthreads();
+    if (pq1 >= 0) {
+      pq0 = ns;
+      pq1 = -1;
+    } else if (pq0 >= 0) {
+      pq1 = ns;
+    } else {
+      pq0 = ns;
+    }
+    last = ns;
+    lastb = b;
+    start = kmin + 1;
+    round = round + 1;
+  }
+  if (pq0 >= 0) rg_update<TR>(g, US, N, wave, li, lk, pq0, pq1 >= 0 ? pq1 : pq0, pq1 >= 0);
+  // G back to memory (right pass: with the mu factors, as k_flip's final flush)
+  const bool mu_w = a.final_flush == 1;
+#pragma unroll
+  for (int tr = 0; tr < TR; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < 2; ++tc) {
+      rg_f4 x = g[tr][tc];
+      if (mu_w) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          x[q] = x[q] * (float)ep;
+          x[q] = x[q] * (float)em;
+        }
+      }
+      *reinterpret_cast<rg_f4 *>(G + 16 * tr + 4 * lk + (size_t)(32 * wave + 16 * tc + li) * N) = x;
+    }
+  for (int bb = tid; bb < N2; bb += nth) sig_t[bb] = sg[bb];
+  if (a.record)
+    for (int q = tid; q < a.dwords; q += nth) a.record[(size_t)chain * a.dwords + q] = s_dec[q];
+  if (tid == 0) {
+    a.nacc[chain] = 0;
+    a.mu_done[chain] = mu;
+    if (!a.gbond) {
+      uint64_t x = (uint64_t)seed0 * (uint64_t)a.apow[N2];
+      a.seed[chain] = (int32_t)(uint32_t)(x & 0x7fffffffull);
+      a.acc_count[2 * chain] += (unsigned long long)tot;
+      a.acc_count[2 * chain + 1] += (unsigned long long)N2;
+    }
+  }
+}

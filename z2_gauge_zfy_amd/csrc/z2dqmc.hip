@@ -16,6 +16,7 @@
 
 #include "../../include/z2dqmc.h"
 #include "flip_kernels.hpp"
+#include "flip_rg.hpp"
 #include "global_kernels.hpp"
 #include "kin_kernels.hpp"
 #include "linalg_kernels.hpp"
@@ -98,6 +99,7 @@ struct z2_ctx {
   long upd_stride = 0;
   int *nacc = nullptr, *mu_done = nullptr;
   int kdelay = 16;  // delay depth of the flip kernel
+  int flip_rg = 1;  // FP32 at N = 256: the flip kernel with G in registers (k_flip_rg)
   int kin_cb = 8, kin_rb = 8;  // kinetic tiles: columns per row-rotation block, rows per column-rotation block
   int kdelay_max = 16;
   size_t lds_max = 65536;
@@ -619,6 +621,12 @@ extern "C" int z2_set_delay(z2_ctx *c, int32_t k) {
   return Z2_OK;
 }
 
+extern "C" int z2_set_flip_variant(z2_ctx *c, int32_t v) {
+  if (!c || (v != Z2_FLIP_DELAYED && v != Z2_FLIP_REGISTER)) return set_err(Z2_EINVAL, "bad flip kernel variant");
+  c->flip_rg = v == Z2_FLIP_REGISTER;
+  return Z2_OK;
+}
+
 extern "C" int z2_set_u_precision(z2_ctx *c, int32_t bits) {
   if (!c || (bits != 32 && bits != 64)) return set_err(Z2_EINVAL, "propagator precision must be 32 or 64 bits");
   if (!c->mixed) return bits == 64 ? Z2_OK : set_err(Z2_EINVAL, "FP64 contexts propagate in FP64");
@@ -1052,8 +1060,22 @@ static int launch_flip(z2_ctx *c, int nt, int descending, int mu_on_read, int fi
   f.gbond = c->gmode ? c->gbond : nullptr;
   f.gmask = c->gmode ? c->gmask : nullptr;
   f.glr = c->gmode ? c->glr : nullptr;
-  size_t sh = flip_lds_bytes(c->N, c->kdelay);
   static_assert(FLIP_THREADS == 512, "k_flip is compiled for 512 threads per block");
+  if constexpr (sizeof(R) == 4) {
+    if (c->N == 256 && c->flip_rg) {
+      // G of the chain in the registers of its workgroup (k_flip_rg); no update
+      // is left pending, the delay depth does not apply
+      const size_t shr = flip_rg_lds_bytes(c->N);
+      if (shr > 65536)
+        HIPC(hipFuncSetAttribute((const void *)k_flip_rg<256>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shr));
+      CHK(prof_begin(c, Z2_K_FLIP));
+      hipLaunchKernelGGL((k_flip_rg<256>), dim3(c->nch), dim3(FLIP_THREADS), shr, c->stream, f);
+      LAUNCH_CHECK();
+      CHK(prof_end(c, Z2_K_FLIP, 0.0));
+      return Z2_OK;
+    }
+  }
+  size_t sh = flip_lds_bytes(c->N, c->kdelay);
   const int mpt = (c->N2 + FLIP_THREADS - 1) / FLIP_THREADS;
   // FP32 at N = 64, 144, 256 (L = 8, 12, 16): kernels compiled for that size
   // (constant distances between the pending vectors and between G's columns;

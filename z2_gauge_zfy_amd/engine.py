@@ -264,6 +264,9 @@ class Engine:
     def set_delay(self, k: int):
         check(lib().z2_set_delay(self._ptr, int(k)))
 
+    def set_flip_variant(self, v: int):
+        check(lib().z2_set_flip_variant(self._ptr, int(v)))
+
     def set_u_precision(self, bits: int):
         check(lib().z2_set_u_precision(self._ptr, int(bits)))
 
