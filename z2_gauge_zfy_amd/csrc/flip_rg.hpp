@@ -6,8 +6,11 @@
 //
 // One workgroup (8 wavefronts) per chain, thread k owns proposal k.  G is
 // loaded once into the wavefronts' registers in the matrix cores' accumulator
-// layout: wavefront w holds columns 32w .. 32w+31 as 16 x 2 tiles of 16 x 16,
-// lane (li, lk) item q of tile (tr, tc) is G(16 tr + 4 lk + q, 32 w + 16 tc + li).
+// layout: wavefront w holds rows R0 = 64 (w / 2) .. R0+63 and columns
+// C0 = 128 (w % 2) .. C0+127 as 4 x 8 tiles of 16 x 16, lane (li, lk) item q of
+// tile (tr, tc) is G(R0 + 16 tr + 4 lk + q, C0 + 16 tc + li); a column of G is
+// spread over four wavefronts and a row over two, so that copying one out
+// takes few stores per lane.
 // Per accepted proposal:
 //   (E) every undecided proposal folds the last accepted update into its 2x2
 //       block (registers) and is evaluated with its cached boson ratio; a
@@ -41,25 +44,27 @@ __host__ __device__ inline size_t flip_rg_lds_bytes(int N) {
 
 // the rank-4 (both = true) or rank-2 update of the register tiles:
 // G(r, c) -= sum_k u_k(r) v_k(c) over the vectors of slots sa (and sb)
-template <int TR>
-__device__ __forceinline__ void rg_update(rg_f4 (&g)[TR][2], const double *US, int N, int wave, int li,
+template <int TRW, int TCW>
+__device__ __forceinline__ void rg_update(rg_f4 (&g)[TRW][TCW], const double *US, int N, int R0, int C0, int li,
                                           int lk, int sa, int sb, bool both) {
   const int cmp = lk & 1;
   const double *U = US + (size_t)4 * N * (lk < 2 ? sa : sb);
   const bool on = lk < 2 || both;
-  float bv[2];
+  float av[TRW], bv[TCW];
 #pragma unroll
-  for (int tc = 0; tc < 2; ++tc) {
-    const float v = (float)(-U[2 * N + 2 * (32 * wave + 16 * tc + li) + cmp]);
+  for (int tr = 0; tr < TRW; ++tr) {
+    const float t = (float)U[2 * (R0 + 16 * tr + li) + cmp];
+    av[tr] = on ? t : 0.f;  // u_k(row)
+  }
+#pragma unroll
+  for (int tc = 0; tc < TCW; ++tc) {
+    const float v = (float)(-U[2 * N + 2 * (C0 + 16 * tc + li) + cmp]);
     bv[tc] = on ? v : 0.f;  // -v_k(col)
   }
 #pragma unroll
-  for (int tr = 0; tr < TR; ++tr) {
-    const float t = (float)U[2 * (16 * tr + li) + cmp];
-    const float av = on ? t : 0.f;  // u_k(row)
+  for (int tr = 0; tr < TRW; ++tr)
 #pragma unroll
-    for (int tc = 0; tc < 2; ++tc) g[tr][tc] = mfma_16x16x4<float>(av, bv[tc], g[tr][tc]);
-  }
+    for (int tc = 0; tc < TCW; ++tc) g[tr][tc] = mfma_16x16x4<float>(av[tr], bv[tc], g[tr][tc]);
 }
 
 // boson ratio of flipping the field s_old of a bond with partner bonds bb and
@@ -81,10 +86,11 @@ __device__ __forceinline__ double rg_dot2(const double *T, int i, const double *
 template <int NC>
 __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   static_assert(NC == 256, "register-resident G: 8 wavefronts x 32 columns");
-  constexpr int N = NC, N2 = 2 * NC, TR = NC / 16, nth = 512;
+  constexpr int N = NC, N2 = 2 * NC, TRW = 4, TCW = 8, nth = 512;  // tiles per wavefront: 4 rows x 8 columns
   const int chain = blockIdx.x;
   const int tid = threadIdx.x;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const int R0 = 64 * (wave >> 1), C0 = 128 * (wave & 1);  // this wavefront's rows and columns of G
   const ChainConst *cp = a.cc + chain;
   const double ep = cp->ep, em = cp->em;
   float *G = a.G + (size_t)chain * N * N;
@@ -112,22 +118,6 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   const double Dm0 = cp->delta[1][0], Dm1 = cp->delta[1][1], Dm2 = cp->delta[1][2], Dm3 = cp->delta[1][3];
   const int mu = a.mu_on_read;
 
-  // G into the registers (with the mu factors of a left pass, as flip_read)
-  rg_f4 g[TR][2];
-#pragma unroll
-  for (int tr = 0; tr < TR; ++tr)
-#pragma unroll
-    for (int tc = 0; tc < 2; ++tc) {
-      rg_f4 x = *reinterpret_cast<const rg_f4 *>(G + 16 * tr + 4 * lk + (size_t)(32 * wave + 16 * tc + li) * N);
-      if (mu) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          x[q] = x[q] * (float)ep;
-          x[q] = x[q] * (float)em;
-        }
-      }
-      g[tr][tc] = x;
-    }
   const uint32_t seed0 = (uint32_t)a.seed[chain] & 0x7fffffffu;
   for (int b = tid; b < N2; b += nth) {
     bp[2 * b] = (short)a.lat.bond_p[2 * b];
@@ -163,6 +153,21 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   const double D0 = rso > 0 ? Dp0 : Dm0, D1 = rso > 0 ? Dp1 : Dm1;
   const double D2 = rso > 0 ? Dp2 : Dm2, D3 = rso > 0 ? Dp3 : Dm3;
 
+  // G into the registers (with the mu factors of a left pass, as flip_read)
+  const float fr1 = mu ? (float)ep : 1.f, fr2 = mu ? (float)em : 1.f;
+  rg_f4 g[TRW][TCW];
+#pragma unroll
+  for (int tr = 0; tr < TRW; ++tr)
+#pragma unroll
+    for (int tc = 0; tc < TCW; ++tc) {
+      rg_f4 x = *reinterpret_cast<const rg_f4 *>(G + R0 + 16 * tr + 4 * lk + (size_t)(C0 + 16 * tc + li) * N);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {  // times 1 (exact) without the factors
+        x[q] = x[q] * fr1;
+        x[q] = x[q] * fr2;
+      }
+      g[tr][tc] = x;
+    }
   int start = 0, round = 0, tot = 0, last = -1, lastb = -1;
   int pq0 = -1, pq1 = -1;  // slots of the accepted updates not yet in the registers, in order
   while (true) {
@@ -215,33 +220,33 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const int P = j ? p2 : p1;
-      if (wave == (P >> 5) && li == (P & 15)) {
-        if ((P >> 4) & 1) {
-#pragma unroll
-          for (int tr = 0; tr < TR; ++tr) *reinterpret_cast<rg_f4 *>(col + j * N + 16 * tr + 4 * lk) = g[tr][1];
-        } else {
-#pragma unroll
-          for (int tr = 0; tr < TR; ++tr) *reinterpret_cast<rg_f4 *>(col + j * N + 16 * tr + 4 * lk) = g[tr][0];
+      // column P: tile column (P >> 4) & 7 of the wavefronts with C0 = 128 (P >> 7), lanes li = P & 15
+      if ((wave & 1) == (P >> 7) && li == (P & 15)) {
+        float *dst = col + j * N + R0 + 4 * lk;
+        switch ((P >> 4) & 7) {
+#define RG_COL_CASE(t)                                                                      \
+  case t:                                                                                   \
+    _Pragma("unroll") for (int tr = 0; tr < TRW; ++tr) *reinterpret_cast<rg_f4 *>(dst + 16 * tr) = g[tr][t]; \
+    break;
+          RG_COL_CASE(0) RG_COL_CASE(1) RG_COL_CASE(2) RG_COL_CASE(3)
+          RG_COL_CASE(4) RG_COL_CASE(5) RG_COL_CASE(6) RG_COL_CASE(7)
+#undef RG_COL_CASE
+          default: break;
         }
       }
-      // row P: item P & 3 of the lanes with lk = (P >> 2) & 3 in tile row P >> 4; the whole
-      // float4 is stored and the item picked by address when it is read
-      rg_f4 x0 = g[0][0], x1 = g[0][1];
-      switch (P >> 4) {
-#define RG_ROW_CASE(t) \
-  case t:              \
-    x0 = g[t][0];      \
-    x1 = g[t][1];      \
+      // row P: item P & 3 of tile row (P >> 4) & 3 of the wavefronts with R0 = 64 (P >> 6), lanes
+      // lk = (P >> 2) & 3; the whole float4 is stored and the item picked by address when read
+      if ((wave >> 1) == (P >> 6) && lk == ((P >> 2) & 3)) {
+        float *dst = row4 + 4 * (j * N + C0 + li);
+        switch ((P >> 4) & 3) {
+#define RG_ROW_CASE(t)                                                                          \
+  case t:                                                                                       \
+    _Pragma("unroll") for (int tc = 0; tc < TCW; ++tc) *reinterpret_cast<rg_f4 *>(dst + 64 * tc) = g[t][tc]; \
     break;
-        RG_ROW_CASE(0) RG_ROW_CASE(1) RG_ROW_CASE(2) RG_ROW_CASE(3) RG_ROW_CASE(4) RG_ROW_CASE(5)
-        RG_ROW_CASE(6) RG_ROW_CASE(7) RG_ROW_CASE(8) RG_ROW_CASE(9) RG_ROW_CASE(10) RG_ROW_CASE(11)
-        RG_ROW_CASE(12) RG_ROW_CASE(13) RG_ROW_CASE(14) RG_ROW_CASE(15)
+          RG_ROW_CASE(0) RG_ROW_CASE(1) RG_ROW_CASE(2) RG_ROW_CASE(3)
 #undef RG_ROW_CASE
-        default: break;
-      }
-      if (lk == ((P >> 2) & 3)) {
-        *reinterpret_cast<rg_f4 *>(row4 + 4 * (j * N + 32 * wave + li)) = x0;
-        *reinterpret_cast<rg_f4 *>(row4 + 4 * (j * N + 32 * wave + 16 + li)) = x1;
+          default: break;
+        }
       }
     }
     // the accepted proposal's 2x2 block as its thread evaluated it, and Prod^{-1}
@@ -337,7 +342,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       TN[2 * N + 2 * (tid - N) + 1] = v1;
     }
     // the two pending updates into the registers (the extraction above has read them)
-    if (pq1 >= 0) rg_update<TR>(g, US, N, wave, li, lk, pq0, pq1, true);
+    if (pq1 >= 0) rg_update<TRW, TCW>(g, US, N, R0, C0, li, lk, pq0, pq1, true);
     __syncthreads();
     if (pq1 >= 0) {
       pq0 = ns;
@@ -352,22 +357,20 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
     start = kmin + 1;
     round = round + 1;
   }
-  if (pq0 >= 0) rg_update<TR>(g, US, N, wave, li, lk, pq0, pq1 >= 0 ? pq1 : pq0, pq1 >= 0);
+  if (pq0 >= 0) rg_update<TRW, TCW>(g, US, N, R0, C0, li, lk, pq0, pq1 >= 0 ? pq1 : pq0, pq1 >= 0);
   // G back to memory (right pass: with the mu factors, as k_flip's final flush)
-  const bool mu_w = a.final_flush == 1;
+  const float fw1 = a.final_flush == 1 ? (float)ep : 1.f, fw2 = a.final_flush == 1 ? (float)em : 1.f;
 #pragma unroll
-  for (int tr = 0; tr < TR; ++tr)
+  for (int tr = 0; tr < TRW; ++tr)
 #pragma unroll
-    for (int tc = 0; tc < 2; ++tc) {
+    for (int tc = 0; tc < TCW; ++tc) {
       rg_f4 x = g[tr][tc];
-      if (mu_w) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          x[q] = x[q] * (float)ep;
-          x[q] = x[q] * (float)em;
-        }
+      for (int q = 0; q < 4; ++q) {  // times 1 (exact) without the factors
+        x[q] = x[q] * fw1;
+        x[q] = x[q] * fw2;
       }
-      *reinterpret_cast<rg_f4 *>(G + 16 * tr + 4 * lk + (size_t)(32 * wave + 16 * tc + li) * N) = x;
+      *reinterpret_cast<rg_f4 *>(G + R0 + 16 * tr + 4 * lk + (size_t)(C0 + 16 * tc + li) * N) = x;
     }
   for (int bb = tid; bb < N2; bb += nth) sig_t[bb] = sg[bb];
   if (a.record)
