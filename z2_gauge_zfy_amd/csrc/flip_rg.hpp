@@ -37,8 +37,8 @@ __host__ __device__ inline size_t flip_rg_lds_bytes(int N) {
   const size_t N2 = 2 * (size_t)N;
   size_t b = sizeof(double) * (4 * 4 * (size_t)N + 4 * N2 + 16);  // 4 update slots, 2x2 per bond, boson
   b += sizeof(float) * (2 + 8) * (size_t)N;  // columns p1, p2; rows p1, p2 as float4 per column
-  b += sizeof(short) * 8 * N2;               // bond sites (2) and plaquette partner bonds (6) per bond
-  b += 2 * N2;                               // current field, sum of the neighbouring slices' fields
+  b += sizeof(short) * 2 * N2;               // bond sites per bond
+  b += N2;                                   // current field
   return (b + 15) & ~(size_t)15;
 }
 
@@ -108,9 +108,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   float *row4 = col + 2 * N;                           // 2 x N x 4: G(p,:) as the float4 of rows
                                                        //   4(p/4) .. 4(p/4)+3 per column
   short *bp = reinterpret_cast<short *>(row4 + 8 * N); // 2N x 2: bond sites
-  short *bbs = bp + 2 * N2;                            // 2N x 6: plaquette partner bonds
-  int8_t *sg = reinterpret_cast<int8_t *>(bbs + 6 * N2);  // 2N : current slice field
-  int8_t *sgs = sg + N2;                                  // 2N : sig(nt-1) + sig(nt+1)
+  int8_t *sg = reinterpret_cast<int8_t *>(bp + 2 * N2);   // 2N : current slice field
   __shared__ int s_kmin[2];
   __shared__ uint32_t s_dec[64];
 
@@ -119,40 +117,21 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   const int mu = a.mu_on_read;
 
   const uint32_t seed0 = (uint32_t)a.seed[chain] & 0x7fffffffu;
-  for (int b = tid; b < N2; b += nth) {
-    bp[2 * b] = (short)a.lat.bond_p[2 * b];
-    bp[2 * b + 1] = (short)a.lat.bond_p[2 * b + 1];
-    for (int q = 0; q < 6; ++q) bbs[6 * b + q] = (short)a.lat.bond_bonds[6 * b + q];
-    sg[b] = sig_t[b];
-    sgs[b] = (int8_t)((int)sig_p[b] + (int)sig_n[b]);
-  }
-  if (tid < 9) bos[tid] = cp->boson[tid];
-  for (int q = tid; q < 64; q += nth) s_dec[q] = 0u;
-  if (tid == 0) {
-    s_kmin[0] = 2 * N2 + 2;
-    s_kmin[1] = 2 * N2 + 2;
-  }
-  const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
-  const int gb = a.gbond ? (a.gmask[(size_t)chain * a.T + (a.nt - 1)] ? a.gbond[chain] : -1) : -1;
-  __syncthreads();
   // this thread's proposal k = tid (2N = 512 threads): bond, sites, partner bonds,
-  // field, random number (draw k+1 of the slice), 2x2 block, boson ratio
+  // field, random number (draw k+1 of the slice), 2x2 block, straight from global
+  // memory so that these loads overlap the tables' copy into LDS
   const int k = tid;
   const int rb_ = a.descending ? (N2 - 1 - k) : k;
-  const int rp1 = bp[2 * rb_], rp2 = bp[2 * rb_ + 1];
+  const int rp1 = a.lat.bond_p[2 * rb_], rp2 = a.lat.bond_p[2 * rb_ + 1];
   int rbb[6];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) rbb[q] = bbs[6 * rb_ + q];
-  const int rso = sg[rb_], rss = sgs[rb_];
+  for (int q = 0; q < 6; ++q) rbb[q] = a.lat.bond_bonds[6 * rb_ + q];
+  const int rso = sig_t[rb_], rss = (int)sig_p[rb_] + (int)sig_n[rb_];
   const double rrn = (double)(uint32_t)(((uint64_t)seed0 * (uint64_t)a.apow[k + 1]) & 0x7fffffffull) / 2147483648.0;
   double r11 = (double)flip_read<float>(G, rp1, rp1, N, mu, ep, em);
   double r22 = (double)flip_read<float>(G, rp2, rp2, N, mu, ep, em);
   double r12 = (double)flip_read<float>(G, rp1, rp2, N, mu, ep, em);
   double r21 = (double)flip_read<float>(G, rp2, rp1, N, mu, ep, em);
-  double rbo = rg_boson(sg, bos, rbb, rso, rss);
-  const double D0 = rso > 0 ? Dp0 : Dm0, D1 = rso > 0 ? Dp1 : Dm1;
-  const double D2 = rso > 0 ? Dp2 : Dm2, D3 = rso > 0 ? Dp3 : Dm3;
-
   // G into the registers (with the mu factors of a left pass, as flip_read)
   const float fr1 = mu ? (float)ep : 1.f, fr2 = mu ? (float)em : 1.f;
   rg_f4 g[TRW][TCW];
@@ -168,6 +147,25 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       }
       g[tr][tc] = x;
     }
+  // lattice tables and the field of the slice into LDS
+  for (int b = tid; b < N2; b += nth) {
+    bp[2 * b] = (short)a.lat.bond_p[2 * b];
+    bp[2 * b + 1] = (short)a.lat.bond_p[2 * b + 1];
+    sg[b] = sig_t[b];
+  }
+  if (tid < 9) bos[tid] = cp->boson[tid];
+  for (int q = tid; q < 64; q += nth) s_dec[q] = 0u;
+  if (tid == 0) {
+    s_kmin[0] = 2 * N2 + 2;
+    s_kmin[1] = 2 * N2 + 2;
+  }
+  const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
+  const int gb = a.gbond ? (a.gmask[(size_t)chain * a.T + (a.nt - 1)] ? a.gbond[chain] : -1) : -1;
+  __syncthreads();
+  double rbo = rg_boson(sg, bos, rbb, rso, rss);
+  const double D0 = rso > 0 ? Dp0 : Dm0, D1 = rso > 0 ? Dp1 : Dm1;
+  const double D2 = rso > 0 ? Dp2 : Dm2, D3 = rso > 0 ? Dp3 : Dm3;
+
   int start = 0, round = 0, tot = 0, last = -1, lastb = -1;
   int pq0 = -1, pq1 = -1;  // slots of the accepted updates not yet in the registers, in order
   while (true) {
