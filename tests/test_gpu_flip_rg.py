@@ -102,3 +102,51 @@ def test_register_kernel_own_decisions_follow_oracle():
     assert e.counters(0)[0] == c.counters()["local_acc"]
     scale = max(1.0, np.abs(c.green()).max())
     assert np.abs(e.green(0) - c.green()).max() / scale < 1e-4
+
+
+def test_register_kernel_wolff_trials_follow_oracle():
+    """Wolff global updates through the register kernel's forced mode (the trial
+    flips one bond at the cluster's slices and accumulates log |det|^2), then a
+    local sweep in each chain's direction, at L = 16: directions, fields, seeds
+    and global counters are the FP64 oracle's, unless a decision falls within
+    FP32 rounding of its random number (the oracle's smallest gap)"""
+    import z2_gauge_zfy_amd as z
+    L, T, beta, w, ng = 16, 20, 1.0, 10, 2
+    specs = [z.ChainSpec(RT=1.0, J=1.0, h=0.5 + 0.2 * k, mu=0.1 * k, Beta=beta, seed=2468 + 2 * 7919 * k)
+             for k in range(2)]
+    e = z.Engine(L, L, T, w, specs, precision=z.Z2_PREC_MIXED)
+    e.set_flip_variant(z.Z2_FLIP_REGISTER)
+    e.init_fields()
+    chains = []
+    for s in specs:
+        p = O.make_params(RT=s.RT, J=s.J, h=s.h, mu=s.mu, Q=s.Q, Nlx=L, Nly=L, Ltrot=T, Beta=beta, Nwrap=w,
+                          absolute=s.absolute, is_global=1, Nglobal=ng)
+        chains.append(O.Chain(O.Model(p), s.seed))
+    e.pre()
+    for c in chains:
+        c.pre()
+        c.is_beta = True
+    for nbc in range(2):
+        e.global_sweep(ng)
+        for c in chains:
+            c.global_sweep()
+        for k, c in enumerate(chains):
+            if e.is_beta(k) != c.is_beta or not np.array_equal(e.sigma(k), c.sigma()):
+                assert c.global_min_gap < 1e-4, f"chain {k} bin {nbc}: global decision away from a near tie"
+                return
+            (ga, gp, cs), _ = e.global_counters(k)
+            kk = c.counters()
+            assert (ga, gp, cs) == (kk["glob_acc"], kk["glob_prop"], kk["cluster_sum"]), (k, nbc)
+        e.sweep(None)
+        for c in chains:
+            c.reset_min_gap()
+            c.sweep()
+        for k, c in enumerate(chains):
+            if not np.array_equal(e.sigma(k), c.sigma()):
+                assert c.min_gap < 1e-5, f"chain {k} bin {nbc}: local decision away from a near tie"
+                return
+            assert e.seed(k) == c.seed
+            scale = max(1.0, np.abs(c.green()).max())
+            assert np.abs(e.green(k) - c.green()).max() / scale < 1e-4
+        print(f"bin {nbc}: directions {[e.is_beta(k) for k in range(2)]}, global {[e.global_counters(k)[0] for k in range(2)]}",
+              flush=True)
