@@ -85,7 +85,7 @@ __device__ __forceinline__ double rg_dot2(const double *T, int i, const double *
 
 template <int NC>
 __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
-  static_assert(NC == 256, "register-resident G: 8 wavefronts x 32 columns");
+  static_assert(NC == 256, "register-resident G: 8 wavefronts x 64 x 128 blocks");
   constexpr int N = NC, N2 = 2 * NC, TRW = 4, TCW = 8, nth = 512;  // tiles per wavefront: 4 rows x 8 columns
   const int chain = blockIdx.x;
   const int tid = threadIdx.x;
