@@ -225,3 +225,109 @@ __global__ void __launch_bounds__(256) k_kin_elem(KinArgs<R> a) {
     M[idx] = kin_epilogue<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km, c);
   }
 }
+
+// Several slices' wraps of a matrix nothing reads between them (the propagators
+// UUL, UUR between two stabilizations): the slices nt0, nt0 + step, ... (ns of
+// them) applied to one tile in LDS in that order, the tile read and written once.
+// Every element sees the operations of ns launches of k_kin_cols / k_kin_rows
+// with the same KinMat (scalar factors before and after the rotations, no flush),
+// including the rounding to R between slices, so the result is theirs bit for bit.
+template <typename R>
+__device__ __forceinline__ double kin_multi_pre(double x, const KinMat<R> &km, const ChainConst &c) {
+  x = apply_fac<R>(x, km.pre1, c);
+  return apply_fac<R>(x, km.pre2, c);
+}
+
+template <typename R, int CB, int NC = 0>
+__global__ void __launch_bounds__(256) k_kin_rows_multi(KinArgs<R> a, int nt0, int ns, int step) {
+  extern __shared__ unsigned char smem_raw[];
+  double *tile = reinterpret_cast<double *>(smem_raw);              // [CB][N]
+  int *tab = reinterpret_cast<int *>(tile + (size_t)CB * a.lat.N);  // [4][N/2]
+  const int N = NC ? NC : a.lat.N, N2 = 2 * N, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
+  const int chain = blockIdx.y;
+  const KinMat<R> km = a.m[blockIdx.z];
+  R *M = km.mat + (size_t)chain * N * N;
+  const int col0 = blockIdx.x * CB;
+  const int ncol = min(CB, N - col0);
+  const ChainConst c = a.cc[chain];
+  for (int cc = 0; cc < ncol; ++cc)
+    for (int r = tid; r < N; r += nt) tile[cc * N + r] = (double)M[r + (size_t)(col0 + cc) * N];
+  const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
+  const int ngrp = half >= nt ? 1 : nt / half;
+  const int items = half * ngrp;
+  for (int s = 0; s < ns; ++s) {
+    const int8_t *sg = a.sig + (size_t)chain * a.T * N2 + (size_t)(nt0 + s * step - 1) * N2;
+    for (int g = 0; g < 4; ++g)
+      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, sg, g, k, km.nflag);
+    for (int cc = 0; cc < ncol; ++cc)
+      for (int r = tid; r < N; r += nt) tile[cc * N + r] = kin_multi_pre<R>(tile[cc * N + r], km, c);
+    __syncthreads();
+    for (int st = 0; st < 8; ++st) {
+      const int *t = tab + kin_group_of_stage(st) * half;
+      for (int it = tid; it < items; it += nt) {
+        const int k = it % half, cg = it / half;
+        const int e = t[k];
+        const int p1 = e & 2047, p2 = (e >> 11) & 2047;
+        const double C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
+        for (int cc = cg; cc < ncol; cc += ngrp) {
+          double x1 = tile[cc * N + p1], x2 = tile[cc * N + p2];
+          double v1 = C * x1 + S * x2;
+          double v2 = S * x1 + C * x2;
+          tile[cc * N + p1] = v1;
+          tile[cc * N + p2] = v2;
+        }
+      }
+      __syncthreads();
+    }
+    for (int cc = 0; cc < ncol; ++cc)
+      for (int r = tid; r < N; r += nt) tile[cc * N + r] = (double)kin_epilogue<R>(tile[cc * N + r], km, c);
+  }
+  for (int cc = 0; cc < ncol; ++cc)
+    for (int r = tid; r < N; r += nt) M[r + (size_t)(col0 + cc) * N] = (R)tile[cc * N + r];
+}
+
+template <typename R, int RB, int NC = 0>
+__global__ void __launch_bounds__(256) k_kin_cols_multi(KinArgs<R> a, int nt0, int ns, int step) {
+  extern __shared__ unsigned char smem_raw[];
+  double *tile = reinterpret_cast<double *>(smem_raw);              // [N][RB]
+  int *tab = reinterpret_cast<int *>(tile + (size_t)RB * a.lat.N);  // [4][N/2]
+  const int N = NC ? NC : a.lat.N, N2 = 2 * N, half = N / 2, nt = blockDim.x, tid = threadIdx.x;
+  const int chain = blockIdx.y;
+  const KinMat<R> km = a.m[blockIdx.z];
+  R *M = km.mat + (size_t)chain * N * N;
+  const int row0 = blockIdx.x * RB;
+  const int nrow = min(RB, N - row0);
+  const ChainConst c = a.cc[chain];
+  const int r = tid % RB, cstep = nt / RB;
+  if (r < nrow)
+    for (int col = tid / RB; col < N; col += cstep) tile[col * RB + r] = (double)M[(row0 + r) + (size_t)col * N];
+  const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
+  for (int s = 0; s < ns; ++s) {
+    const int8_t *sg = a.sig + (size_t)chain * a.T * N2 + (size_t)(nt0 + s * step - 1) * N2;
+    for (int g = 0; g < 4; ++g)
+      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, sg, g, k, km.nflag);
+    if (r < nrow)
+      for (int col = tid / RB; col < N; col += cstep) tile[col * RB + r] = kin_multi_pre<R>(tile[col * RB + r], km, c);
+    __syncthreads();
+    for (int st = 0; st < 8; ++st) {
+      const int *t = tab + kin_group_of_stage(st) * half;
+      if (r < nrow)
+        for (int k = tid / RB; k < half; k += cstep) {
+          const int e = t[k];
+          const int p1 = e & 2047, p2 = (e >> 11) & 2047;
+          const double C = (e >> 22) ? Cp : Cm, S = (e >> 22) ? Sp : Sm;
+          double x1 = tile[p1 * RB + r], x2 = tile[p2 * RB + r];
+          double u1 = x1 * C + x2 * S;
+          double u2 = x1 * S + x2 * C;
+          tile[p1 * RB + r] = u1;
+          tile[p2 * RB + r] = u2;
+        }
+      __syncthreads();
+    }
+    if (r < nrow)
+      for (int col = tid / RB; col < N; col += cstep)
+        tile[col * RB + r] = (double)kin_epilogue<R>(tile[col * RB + r], km, c);
+  }
+  if (r < nrow)
+    for (int col = tid / RB; col < N; col += cstep) M[(row0 + r) + (size_t)col * N] = (R)tile[col * RB + r];
+}

@@ -81,6 +81,9 @@ struct z2_ctx {
   // nch_all: chains of the context
   int N = 0, N2 = 0, T = 0, w = 0, Nst = 0, nch = 0, nch_all = 0, Lx = 0, Ly = 0;
   hipStream_t stream = nullptr;
+  // propagator wraps deferred to the next stabilization: ud_n slices nt0, nt0 + step,
+  // ... of UUL (ud_cols = 1, left pass) or UUR (0, right pass) not applied yet
+  int ud_n = 0, ud_nt0 = 0, ud_step = 0, ud_cols = 0;
   std::vector<z2_chain_desc> desc;
   std::vector<ChainConst> hcc;
   ChainConst *dcc = nullptr;
@@ -285,6 +288,8 @@ static void register_fields(z2_ctx *c) {
 
 // A contiguous range [c0, c0 + cnt) of storage slots seen by the launch code as
 // the whole context (every per-chain pointer moved by c0 chains), on `stream`.
+static int u_flush(z2_ctx *c);
+
 struct ChainRange {
   z2_ctx *c;
   int c0, nsave;
@@ -628,6 +633,7 @@ extern "C" int z2_set_flip_variant(z2_ctx *c, int32_t v) {
 }
 
 extern "C" int z2_set_u_precision(z2_ctx *c, int32_t bits) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || (bits != 32 && bits != 64)) return set_err(Z2_EINVAL, "propagator precision must be 32 or 64 bits");
   if (!c->mixed) return bits == 64 ? Z2_OK : set_err(Z2_EINVAL, "FP64 contexts propagate in FP64");
   if (c->have_pre) return set_err(Z2_ESTATE, "set the propagator precision before z2_pre");
@@ -722,6 +728,7 @@ static int upload_sigma(z2_ctx *c, int chain, const double *sigma) {
 }
 
 extern "C" int z2_init_fields(z2_ctx *c) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   std::vector<double> sig;
   std::vector<int32_t> seeds(c->nch);
@@ -739,6 +746,7 @@ extern "C" int z2_init_fields(z2_ctx *c) {
 }
 
 extern "C" int z2_set_sigma(z2_ctx *c, int32_t chain, const double *sigma, int32_t seed) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || !sigma || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
   chain = c->phys[chain];  // storage slot
   CHK(upload_sigma(c, chain, sigma));
@@ -765,6 +773,7 @@ extern "C" int z2_get_lambda(z2_ctx *c, int32_t chain, double *lambda) {
 }
 
 extern "C" int z2_set_lambda(z2_ctx *c, int32_t chain, const double *lambda) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || !lambda || chain < 0 || chain >= c->nch_all) return set_err(Z2_EINVAL, "bad argument");
   for (int i = 0; i < c->N; ++i)
     if (lambda[i] != 1.0 && lambda[i] != -1.0) return set_err(Z2_EINVAL, "lambda values must be +-1");
@@ -804,6 +813,7 @@ static int conf_transfer_host(const z2_ctx *c, int LxT, int LyT, int TT, double 
 
 extern "C" int z2_set_sigma_therm(z2_ctx *c, int32_t chain, int32_t NlxTherm, int32_t NlyTherm, int32_t LtrotTherm,
                                   const double *sigma_therm, const double *lambda_therm, int32_t seed) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || !sigma_therm || !lambda_therm || chain < 0 || chain >= c->nch_all)
     return set_err(Z2_EINVAL, "bad argument");
   std::vector<double> sig, lam(c->N);
@@ -814,6 +824,7 @@ extern "C" int z2_set_sigma_therm(z2_ctx *c, int32_t chain, int32_t NlxTherm, in
 }
 
 extern "C" int z2_init_fields_therm(z2_ctx *c, int32_t NlxTherm, int32_t NlyTherm, int32_t LtrotTherm) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (NlxTherm < 1 || NlyTherm < 1 || LtrotTherm < 1) return set_err(Z2_EINVAL, "bad thermalisation size");
   std::vector<double> sig_t, lam_t((size_t)NlxTherm * NlyTherm);
@@ -1651,6 +1662,7 @@ static inline size_t slot(const z2_ctx *c, int st) { return (size_t)st * c->nch_
 static inline size_t slotd(const z2_ctx *c, int st) { return (size_t)st * c->nch_all * c->N; }
 
 extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect preortho time slice %d", nt);
   if (!c->have_fields) return set_err(Z2_ESTATE, "fields not initialised (z2_init_fields / z2_set_sigma)");
   const int st = nt / c->w;
@@ -1676,6 +1688,7 @@ extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
 }
 
 extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect ortholeft time slice %d", nt);
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
@@ -1697,6 +1710,7 @@ extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
 }
 
 extern "C" int z2_wrap_R(z2_ctx *c, int32_t nt, int32_t flag) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect orthoright time slice %d", nt);
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
@@ -1733,19 +1747,82 @@ static int kin_u64(z2_ctx *c, int nt, double *U, int cols, int nflag, int pre, i
   return cols ? launch_cols<double>(c, d, 1) : launch_rows<double>(c, d, 1);
 }
 
+// ns slices' wraps of one propagator in one launch (k_kin_cols_multi / k_kin_rows_multi)
+template <typename R, int NC>
+static int launch_u_multi_n(z2_ctx *c, KinArgs<R> &a, int cols, int nt0, int ns, int step) {
+  const size_t tab = sizeof(int) * 2 * (size_t)c->N;
+  if (cols) {
+    const int RB = c->kin_rb;
+    const size_t sh = sizeof(double) * RB * c->N + tab;
+    const void *fn = RB == 32 ? (const void *)k_kin_cols_multi<R, 32, NC> : (const void *)k_kin_cols_multi<R, 8, NC>;
+    if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    dim3 grid((c->N + RB - 1) / RB, c->nch, 1);
+    if (RB == 32) hipLaunchKernelGGL((k_kin_cols_multi<R, 32, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
+    else hipLaunchKernelGGL((k_kin_cols_multi<R, 8, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
+  } else {
+    const int CB = c->kin_cb;
+    const size_t sh = sizeof(double) * CB * c->N + tab;
+    const void *fn = CB == 16 ? (const void *)k_kin_rows_multi<R, 16, NC> : (const void *)k_kin_rows_multi<R, 8, NC>;
+    if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    dim3 grid((c->N + CB - 1) / CB, c->nch, 1);
+    if (CB == 16) hipLaunchKernelGGL((k_kin_rows_multi<R, 16, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
+    else hipLaunchKernelGGL((k_kin_rows_multi<R, 8, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
+  }
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+template <typename R>
+static int launch_u_multi(z2_ctx *c, R *U, int cols, int nt0, int ns, int step) {
+  KinArgs<R> a = kin_args<R>(c, nt0);
+  // UUL <- (UUL*ep) B per left-pass slice, UUR <- (B UUR)*ep per right-pass slice
+  a.m[0] = cols ? kmat<R>(U, +1, 0, FAC_EP) : kmat<R>(U, +1, 0, 0, 0, 0, 0, FAC_EP);
+  const int kind = cols ? Z2_K_KIN_COLS : Z2_K_KIN_ROWS;
+  CHK(prof_begin(c, kind));
+  switch (c->N) {
+    case 64: CHK((launch_u_multi_n<R, 64>(c, a, cols, nt0, ns, step))); break;
+    case 144: CHK((launch_u_multi_n<R, 144>(c, a, cols, nt0, ns, step))); break;
+    case 256: CHK((launch_u_multi_n<R, 256>(c, a, cols, nt0, ns, step))); break;
+    default: CHK((launch_u_multi_n<R, 0>(c, a, cols, nt0, ns, step)));
+  }
+  CHK(prof_end(c, kind, 2.0 * sizeof(R) * c->N * c->N * (double)c->nch));
+  return Z2_OK;
+}
+
+// apply the deferred propagator wraps (before anything reads or writes the
+// propagators or changes a field the deferred slices were made from)
+static int u_flush(z2_ctx *c) {
+  if (!c->ud_n) return Z2_OK;
+  const int ns = c->ud_n;
+  c->ud_n = 0;
+  if (!c->mixed || c->u64)
+    return launch_u_multi<double>(c, c->ud_cols ? c->UUL : c->UUR, c->ud_cols, c->ud_nt0, ns, c->ud_step);
+  return launch_u_multi<float>(c, c->ud_cols ? c->UULf : c->UURf, c->ud_cols, c->ud_nt0, ns, c->ud_step);
+}
+
+// the propagator wrap of slice nt (left pass: cols = 1, slices in decreasing order;
+// right pass: rows, increasing), deferred: between two stabilizations nothing reads
+// UUL or UUR, so an interval's wraps run as one launch at its end
+static int u_defer(z2_ctx *c, int nt, int cols) {
+  const int step = cols ? -1 : 1;
+  if (c->ud_n && (c->ud_cols != cols || nt != c->ud_nt0 + c->ud_n * c->ud_step)) CHK(u_flush(c));
+  if (!c->ud_n) {
+    c->ud_nt0 = nt;
+    c->ud_cols = cols;
+    c->ud_step = step;
+  }
+  ++c->ud_n;
+  return Z2_OK;
+}
+
 template <typename R>
 static int slice_L_impl(z2_ctx *c, int nt) {
   CHK(launch_flip<R>(c, nt, 1, 1));
   KinArgs<R> a = kin_args<R>(c, nt);
   a.m[0] = kmat<R>(PB<R>::G(c), +1, 1, FAC_EP, FAC_EM);  // G <- (flush((G*ep)*em)) B
   a.m[0].pre_cond = 1;                                    // mu already in G if the flip kernel flushed
-  if (u_with_g<R>(c)) {
-    a.m[1] = kmat<R>(PB<R>::UUL(c), +1, 0, FAC_EP);      // UUL <- (UUL*ep) B
-    CHK(launch_cols<R>(c, a, 2));
-  } else {
-    CHK(launch_cols<R>(c, a, 1));
-    CHK(kin_u64(c, nt, c->UUL, 1, +1, FAC_EP, 0));
-  }
+  CHK(launch_cols<R>(c, a, 1));
+  CHK(u_defer(c, nt, 1));                                 // UUL <- (UUL*ep) B
   a.m[0] = kmat<R>(PB<R>::G(c), -1);                     // G <- B^-1 G
   CHK(launch_rows<R>(c, a, 1));
   return Z2_OK;
@@ -1770,10 +1847,7 @@ static int slice_R_impl(z2_ctx *c, int nt) {
     a.m[0] = kmat<R>(PB<R>::G(c), 0, 1, 0, 0, FAC_EP, FAC_EM);
     CHK(launch_elem_kin<R>(c, a, 1));
   }
-  if (!u_with_g<R>(c)) return kin_u64(c, nt, c->UUR, 0, +1, 0, FAC_EP);
-  a.m[0] = kmat<R>(PB<R>::UUR(c), +1, 0, 0, 0, 0, 0, FAC_EP);  // UUR <- (B UUR)*ep
-  CHK(launch_rows<R>(c, a, 1));
-  return Z2_OK;
+  return u_defer(c, nt, 0);  // UUR <- (B UUR)*ep
 }
 
 extern "C" int z2_slice_L(z2_ctx *c, int32_t nt) {
@@ -1928,6 +2002,7 @@ static int stab_green_big(z2_ctx *c) {
 
 // Wrap_tau, src/stabilization.f90:478-514
 static int wrap_tau(z2_ctx *c, int nt) {
+  CHK(u_flush(c));
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
   CHK(dcopy(c, c->dUUL, c->ULl + slot(c, st), nn));
@@ -1957,6 +2032,7 @@ static int obs_tau(z2_ctx *c, int nt) {
 // Dyn_reset, src/dynamics.f90:41-48: the second right propagator from the current
 // one, G(0,0) = G(t,t) = G(t,0) = G, G(0,t) = -(I - G)
 static int dyn_reset(z2_ctx *c) {
+  CHK(u_flush(c));
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
   CHK(dcopy(c, c->dUUR, c->UUR, nn));
   CHK(dcopy(c, c->dVUR, c->VUR, nn));
@@ -1975,6 +2051,7 @@ static int dyn_reset(z2_ctx *c) {
 // Dyn_reset + Dyn%sweep_R, src/dynamics.f90:41-62: starts from the current G and
 // right propagator, steps G(t,0), G(0,t), G(t,t) and U_R through all slices.
 static int dyn_sweep(z2_ctx *c) {
+  CHK(u_flush(c));
   CHK(dyn_reset(c));
   for (int nt = 1; nt <= c->T; ++nt) {
     CHK(obs_tau(c, nt));
@@ -1993,6 +2070,7 @@ static int dyn_sweep(z2_ctx *c) {
 }
 
 extern "C" int z2_dyn_measure(z2_ctx *c) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->tau_on) return set_err(Z2_ESTATE, "time-displaced measurement not enabled (z2_enable_tau)");
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_dyn_measure");
@@ -2029,6 +2107,7 @@ static int check_slice(z2_ctx *c, int nt) {
 
 // propK_pre, src/multiply.f90:8-15: UUR <- B(nt) UUR
 extern "C" int z2_prop_K_pre(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(check_slice(c, nt));
   if (c->mixed && !c->u64) {
     KinArgs<float> a = kin_args<float>(c, nt);
@@ -2042,6 +2121,7 @@ extern "C" int z2_prop_K_pre(z2_ctx *c, int32_t nt) {
 
 // propT_pre, src/multiply.f90:65-77: UUR <- UUR exp(dtau mu)
 extern "C" int z2_prop_T_pre(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(check_slice(c, nt));
   if (c->mixed && !c->u64) return mu_elem<float>(c, nt, c->UURf, FAC_EP, FAC_NONE, (float *)nullptr, 0);
   return mu_elem<double>(c, nt, c->UUR, FAC_EP, FAC_NONE, (double *)nullptr, 0);
@@ -2049,6 +2129,7 @@ extern "C" int z2_prop_T_pre(z2_ctx *c, int32_t nt) {
 
 // propT_L, src/multiply.f90:79-93: G <- (G e^{+}) e^{-}, UUL <- UUL e^{+}
 extern "C" int z2_prop_T_L(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(check_slice(c, nt));
   if (c->mixed && c->u64) {
     CHK(mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, (float *)nullptr, 0));
@@ -2060,6 +2141,7 @@ extern "C" int z2_prop_T_L(z2_ctx *c, int32_t nt) {
 
 // propT_R, src/multiply.f90:95-108: G <- (G e^{+}) e^{-}, UUR <- UUR e^{+}
 extern "C" int z2_prop_T_R(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(check_slice(c, nt));
   if (c->mixed && c->u64) {
     CHK(mu_elem<float>(c, nt, c->Gf, FAC_EP, FAC_EM, (float *)nullptr, 0));
@@ -2108,11 +2190,13 @@ static int localK_R_impl(z2_ctx *c, int nt) {
 }
 
 extern "C" int z2_localK_prop_L(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(check_slice(c, nt));
   return c->mixed ? localK_L_impl<float>(c, nt) : localK_L_impl<double>(c, nt);
 }
 
 extern "C" int z2_localK_prop_R(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(check_slice(c, nt));
   return c->mixed ? localK_R_impl<float>(c, nt) : localK_R_impl<double>(c, nt);
 }
@@ -2126,6 +2210,7 @@ static int need_tau(z2_ctx *c) {
 // propgrK_R, src/multiply.f90:43-63: G(t,0) <- B G(t,0), G(0,t) <- G(0,t) B^{-1},
 // G(t,t) <- B G(t,t) B^{-1}, U_R <- B U_R on the time-displaced state (FP64)
 extern "C" int z2_propgr_K_R(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(need_tau(c));
   CHK(check_slice(c, nt));
   KinArgs<double> a = kin_args<double>(c, nt);
@@ -2140,6 +2225,7 @@ extern "C" int z2_propgr_K_R(z2_ctx *c, int32_t nt) {
 
 // propgrT_R, src/multiply.f90:110-130: G(t,0) e^{+}, G(0,t) e^{-}, (G(t,t) e^{+}) e^{-}, U_R e^{+}
 extern "C" int z2_propgr_T_R(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(need_tau(c));
   CHK(check_slice(c, nt));
   KinArgs<double> a = kin_args<double>(c, nt);
@@ -2152,6 +2238,7 @@ extern "C" int z2_propgr_T_R(z2_ctx *c, int32_t nt) {
 
 // Dyn%reset, src/dynamics.f90:41-48 (Propagator%asgn, Propgr_reset src/process_matrix.f90:88-96)
 extern "C" int z2_dyn_reset(z2_ctx *c) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(need_tau(c));
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_dyn_reset");
   if (c->left_done == 0)
@@ -2183,6 +2270,7 @@ extern "C" int z2_obs_tau_ave(z2_ctx *c, int32_t nobst) {
 
 // Wrap_tau, src/stabilization.f90:478-514
 extern "C" int z2_wrap_tau(z2_ctx *c, int32_t nt) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   CHK(need_tau(c));
   if (nt < 1 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect orthobig time slice %d", nt);
   return wrap_tau(c, nt);
@@ -2247,6 +2335,7 @@ static int h_zero_sync(z2_ctx *c) {
 }
 
 extern "C" int z2_pre(z2_ctx *c) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->have_fields) return set_err(Z2_ESTATE, "fields not initialised (z2_init_fields / z2_set_sigma)");
   CHK(h_zero_sync(c));
@@ -2381,6 +2470,7 @@ static int reorder(z2_ctx *c, const std::vector<int> &perm) {
 
 // stores the chains with is_beta = 1 first; *n1 = their number
 static int partition(z2_ctx *c, int *n1) {
+  CHK(u_flush(c));
   std::vector<int> perm;
   for (int j = 0; j < c->nch_all; ++j)
     if (c->isb[j]) perm.push_back(j);
@@ -2411,6 +2501,7 @@ static int sweep_prologue(z2_ctx *c, int toggle) {
 }
 
 static int sweep_epilogue(z2_ctx *c, int toggle, int nobst) {
+  CHK(u_flush(c));
   CHK(obs_ave(c));
   if (toggle && nobst > 0) {
     hipLaunchKernelGGL(k_scale, dim3(256), dim3(256), 0, c->stream, c->otau, (size_t)c->nch * TAU_NOBS * c->T * c->N,
@@ -2459,6 +2550,7 @@ extern "C" int z2_sweep_tau(z2_ctx *c, int32_t is_beta, int32_t toggle) {
     sweep_steps(c, g == 0 ? 1 : 0, toggle, v);
     ChainRange r(c, c0, cnt, c->stream);
     CHK(run_steps(c, v, &nob[g]));
+    CHK(u_flush(c));  // in this chain range
   }
   const int nobst = nob[0] > nob[1] ? nob[0] : nob[1];
   CHK(obs_ave(c));
@@ -2546,6 +2638,7 @@ static void global_steps(const z2_ctx *c, int is_beta, std::vector<Step> &v) {
 // Global_sweep (src/global_update.f90:313-337) for every chain, each in its own
 // direction; an accepted trial reverses the chain's is_beta.
 extern "C" int z2_global_sweep(z2_ctx *c, int32_t nglobal) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c || nglobal < 1) return set_err(Z2_EINVAL, "null ctx or Nglobal < 1");
   if (!c->have_pre) return set_err(Z2_ESTATE, "z2_pre must run before z2_global_sweep");
   if (c->force) return set_err(Z2_ESTATE, "global sweep during a replay");
@@ -2568,6 +2661,7 @@ extern "C" int z2_global_sweep(z2_ctx *c, int32_t nglobal) {
         int rc = run_steps(c, v, nullptr);
         c->gmode = false;
         CHK(rc);
+        CHK(u_flush(c));  // in this chain range
         hipLaunchKernelGGL(k_wolff_accept, gr, b1, 0, c->stream, glob_args(c));
         LAUNCH_CHECK();
       }
@@ -2600,6 +2694,7 @@ extern "C" int z2_get_global_counters(z2_ctx *c, int32_t chain, uint64_t out[3],
 
 // LocalK_therm sweep (src/localK.f90:400-420), the boson-only warm-up
 extern "C" int z2_therm(z2_ctx *c) {
+  if (c) CHK(u_flush(c));  // deferred propagator wraps first
   if (!c) return set_err(Z2_EINVAL, "null ctx");
   if (!c->have_fields) return set_err(Z2_ESTATE, "fields not initialised (z2_init_fields / z2_set_sigma)");
   CHK(h_zero_sync(c));  // Local_sweep_reset (src/local_sweep.f90:63-69)
