@@ -45,8 +45,8 @@ def _c3():
 
 
 CONFIGS = {
-    "c4": dict(chains=_grid(16, 200, 10, 20.0, 0.0, (0.4, 0.6, 0.8, 1.0), 64), precision="both", groups=1,
-               extra_groups=2,
+    "c4": dict(chains=_grid(16, 200, 10, 20.0, 0.0, (0.4, 0.6, 0.8, 1.0), 64), precision="both", groups=2,
+               extra_groups=1,
                desc="64 chains, L=16 (N=256), beta=20, Ltrot=200, Nwrap=10, h in [0.4, 0.6, 0.8, 1.0] x 16, "
                     "J=1, t=1, mu=0"),
     "c3": dict(chains=_c3(), precision="fp64", groups=1, tune=5e-3,
@@ -414,8 +414,8 @@ def main():
     if "mixed" in res and "fp64" in res:
         out["mixed_over_fp64"] = res["mixed"]["sweeps_per_s"] / res["fp64"]["sweeps_per_s"]
     if W.get("extra_groups") and a.groups is None and "mixed" in res:
-        # the same chains split over more contexts (HIP streams), timed on their own:
-        # one group's flip kernel overlaps another's wraps and stabilization
+        # the same chains in another number of contexts (HIP streams), timed on their own:
+        # with several, one group's kernels fill the gaps and idle CUs of another's
         eg = W["extra_groups"]
         r2 = run_mode(a.config, "mixed", a.steps, a.warmup, "", a.delay, (), groups=eg, exec_mode=a.exec)
         out[f"mixed_{eg}groups_sweeps_per_s"] = r2["sweeps_per_s"]

@@ -142,6 +142,37 @@ __device__ __forceinline__ void qrb_wave_sum_n(double (&v)[C]) {
   for (int q = 0; q < C; ++q) v[q] = qrb_readlane(v[q], 63);
 }
 
+// The sums of v[Q..C-1] only (the same trees as qrb_wave_sum_n), for Q = qa, the
+// first entry still needed; qa >= C: none
+template <int C, int Q>
+__device__ __forceinline__ void qrb_wave_sum_tail(double (&v)[C]) {
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] += qrb_dpp<0xB1>(v[q]);
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] += qrb_dpp<0x4E>(v[q]);
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] += qrb_dpp<0x141>(v[q]);
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] += qrb_dpp<0x140>(v[q]);
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] += qrb_dpp_rows<0x142, 0xA>(v[q]);
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] += qrb_dpp_rows<0x143, 0xC>(v[q]);
+#pragma unroll
+  for (int q = Q; q < C; ++q) v[q] = qrb_readlane(v[q], 63);
+}
+
+template <int C, int Q = 0>
+__device__ __forceinline__ void qrb_wave_sum_from(double (&v)[C], int qa) {
+  if constexpr (Q < C) {
+    if (qa == Q) {
+      qrb_wave_sum_tail<C, Q>(v);
+      return;
+    }
+    qrb_wave_sum_from<C, Q + 1>(v, qa);
+  }
+}
+
 // Householder factorization of a panel (m x w, w <= NB) in registers: reflectors
 // below the diagonal, alpha kept on it (beta in s_beta), tau in s_tau.  Wavefront w owns columns
 // c = w + 8 q, lane l rows r = l + 64 t (RPL = ceil(N / 64)).  One barrier per
@@ -242,7 +273,10 @@ __device__ void qrb_panel_reg(QrbLds &L, const double *Wp, int ldW, const int *c
       const int r = lane + 64 * t;
       vr[t] = qrb_ld(vb, (size_t)r, r < m);
     }
-    if (tk != 0.0) {
+    // this wavefront's columns c = wave + NW q still to be updated: q >= qa (c > k);
+    // the sums of the others are skipped (uniform per wavefront)
+    const int qa = k < wave ? 0 : (k - wave) / NW + 1;
+    if (tk != 0.0 && qa < CPW) {
       double d[CPW];
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
@@ -252,7 +286,7 @@ __device__ void qrb_panel_reg(QrbLds &L, const double *Wp, int ldW, const int *c
 #pragma unroll
           for (int t = 0; t < RPL; ++t) d[q] += vr[t] * x[q][t];
       }
-      qrb_wave_sum_n<CPW>(d);
+      qrb_wave_sum_from<CPW>(d, qa);
 #pragma unroll
       for (int q = 0; q < CPW; ++q) {
         const int c = wave + NW * q;
