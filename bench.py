@@ -108,13 +108,16 @@ def step_roofline(shapes, precision, ms_per_step):
             "ms_per_step": ms_per_step, "frac": t_roof * 1e3 / ms_per_step}
 
 
-def measured_traffic(kind, cfg, precision):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any"""
+def measured_traffic(kind, cfg, precision, chains_per_launch):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any,
+    scaled from the chains per launch it was measured at to chains_per_launch"""
     path = os.path.join(ROOT, "profiles", f"traffic_{kind}_{cfg}_{precision}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        j = json.load(f)
+    b, at = j.get("bytes_per_launch"), j.get("chains_per_launch")
+    return b * chains_per_launch / at if (b is not None and at) else b
 
 
 def _cpu_worker(args):
@@ -245,8 +248,28 @@ def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay, extra=(), sn
         for g in range(ngroups):
             b.group(g).profile_kernel(kind)
 
+    def spans_union():
+        """(union of the timed launches' intervals over the groups, their sum) in ms"""
+        ivs = []
+        for g in range(ngroups):
+            st, en = b.group(g).profile_spans(b.group(0))
+            ivs.extend(zip(st.tolist(), en.tolist()))
+        ivs.sort()
+        union, cur0, cur1 = 0.0, None, None
+        for s0, s1 in ivs:
+            if cur1 is None or s0 > cur1:
+                if cur1 is not None:
+                    union += cur1 - cur0
+                cur0, cur1 = s0, s1
+            else:
+                cur1 = max(cur1, s1)
+        if cur1 is not None:
+            union += cur1 - cur0
+        return union, sum(e - s for s, e in ivs)
+
     def profile_read(kind, c0, c1):
         ms, nl, wk = 0.0, 0, 0.0
+        span = spans_union() if ngroups > 1 else None
         for g in range(ngroups):
             e = b.group(g)
             t, n = e.profile_read()
@@ -254,7 +277,7 @@ def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay, extra=(), sn
             e.profile_kernel("")
         if kind == "flip":  # 4 N^2 flops per accepted rank-2 update
             wk = sum(4.0 * Ns[k] ** 2 * (c1[k][0] - c0[k][0]) for k in range(nch))
-        return ms, nl, wk
+        return ms, nl, wk, span
 
     c0 = accepted()
     live = roofline_kernel and not graph
@@ -270,8 +293,9 @@ def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay, extra=(), sn
     dt = time.perf_counter() - t0
     c1 = accepted()
     kms = klaunch = kwork = 0
+    kspan = None
     if live:
-        kms, klaunch, kwork = profile_read(roofline_kernel, c0, c1)
+        kms, klaunch, kwork, kspan = profile_read(roofline_kernel, c0, c1)
     others = {}
     kinds = list(extra) + ([roofline_kernel] if roofline_kernel and graph else [])
     if kinds:
@@ -283,7 +307,7 @@ def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay, extra=(), sn
         b.sweep(True)
         b.sync()
         t_sw = time.perf_counter() - t1
-        ms, nl, wk = profile_read(kind, a0, accepted())
+        ms, nl, wk, _ = profile_read(kind, a0, accepted())
         others[kind] = (ms, nl, wk, ms / 1e3 / t_sw)
         progress(f"{precision} extra sweep timing {kind} done")
     if roofline_kernel and graph:
@@ -296,7 +320,7 @@ def run_mode(cfg, precision, steps, warmup, roofline_kernel, delay, extra=(), sn
         nw = {(g["L"], g["Ltrot"]): g["Nwrap"] for g in tuning["groups"]}
         shapes = [(c.L * c.L, c.Ltrot, nw.get((c.L, c.Ltrot), c.Nwrap)) for c in chains]
     return dict(dt=dt, sweeps_per_s=nch * steps / dt, flips_per_s=prop / dt, acc_per_s=acc / dt,
-                acceptance=acc / max(1, prop), kms=kms, klaunch=klaunch, kwork=kwork, accepted=acc,
+                acceptance=acc / max(1, prop), kms=kms, klaunch=klaunch, kwork=kwork, kspan=kspan, accepted=acc,
                 others=others, snapshot=snap, groups=ngroups, exec="graph" if graph else "eager",
                 roofline_live=bool(live), tuning=tuning, shapes=shapes)
 
@@ -430,15 +454,23 @@ def main():
                                                if "fp64" in res else None)
     Nrep = chains[0].L ** 2
     rf = roofline(a.roofline_kernel, Nrep, head["kms"], head["klaunch"], head["kwork"], head_name,
-                  measured_traffic(a.roofline_kernel, a.config, head_name))
+                  measured_traffic(a.roofline_kernel, a.config, head_name, nch / head["groups"]))
     rf["kernel_share_of_step"] = (head["kms"] / 1e3) / head["dt"] if head["roofline_live"] else None
+    if head.get("kspan"):
+        # several stream groups: their launches overlap; the rate over the time any
+        # launch of the class runs (union of the groups' launch intervals)
+        union_ms, sum_ms = head["kspan"]
+        rf["launches_in_flight"] = sum_ms / union_ms
+        rf["achieved_all_groups"] = head["kwork"] / (union_ms / 1e3) / (1e12 if rf["unit"] == "TFLOP/s" else 1e9)
+        rf["frac_all_groups"] = rf["achieved_all_groups"] / rf["peak"]
+        rf["union_share_of_step"] = (union_ms / 1e3) / head["dt"]
     rf["measured_in"] = "timed sweeps" if head["roofline_live"] else "one extra eager sweep after the timed ones"
     out["roofline"] = rf
     out["roofline_step"] = {k: step_roofline(r["shapes"], k, r["dt"] / a.steps * 1e3) for k, r in res.items()}
     for k, r in res.items():
         other = []
         for kind, (ms, nl, wk, share) in r["others"].items():
-            ro = roofline(kind, Nrep, ms, nl, wk, k, measured_traffic(kind, a.config, k))
+            ro = roofline(kind, Nrep, ms, nl, wk, k, measured_traffic(kind, a.config, k, nch / r["groups"]))
             ro["kernel_share_of_sweep"] = share
             other.append(ro)
         out["roofline_other" if k == head_name else f"roofline_other_{k}"] = other

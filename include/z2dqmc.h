@@ -330,6 +330,12 @@ int z2_batch_load_tuning(z2_batch *b, const char *path);
 #define Z2_K_QT_APPLY 10  /* Q^T applied as block reflectors in the G recompute */
 int z2_profile_kernel(z2_ctx *ctx, int32_t kind);
 int z2_profile_read(z2_ctx *ctx, double *total_ms, int64_t *launches);
+/* start and end (ms) of the first min(cap, launches) timed launches, measured
+ * from the moment z2_profile_kernel last ran on ref (ctx itself or another context
+ * on the same device, profiled before ctx; a later z2_profile_kernel on ref moves
+ * that origin): the union over contexts shows how many launches of the class run
+ * at once; *n = entries written */
+int z2_profile_spans(z2_ctx *ctx, z2_ctx *ref, double *start_ms, double *end_ms, int64_t cap, int64_t *n);
 /* algorithmic work of the launches timed since z2_profile_kernel: flops for
  * QR (4/3 n^3), explicit Q (4/3 n^3), Q^T application (2 n^3), GEMM (2 m n k),
  * TRSM (n^3); bytes for the kinetic and observable kernels (read + write of

@@ -163,3 +163,33 @@ def test_autotune_persists_and_reloads(tmp_path):
         _same(a, b, k, f"chain {k}")
     with pytest.raises(z.Z2Error, match="before z2_batch_pre"):
         a.autotune()
+
+
+def test_profile_spans_lay_groups_on_one_time_line():
+    """z2_profile_spans: every timed flip launch of two stream groups, measured from
+    the first group's z2_profile_kernel; per group one launch per slice step, in
+    order, and their durations sum to z2_profile_read's total"""
+    import z2_gauge_zfy_amd as z
+    T = 20
+    chains = [z.BatchChain(4, T, 5, z.ChainSpec(h=0.4 + 0.2 * k, Beta=2.0, seed=777 + 2 * 7919 * k)) for k in range(4)]
+    b = z.Batch(chains, precision=z.Z2_PREC_MIXED, max_group=2)
+    assert b.info() == (4, 2)
+    b.init_fields()
+    b.pre()
+    b.sync()
+    for g in range(2):
+        b.group(g).profile_kernel("flip")
+    b.sweep(True)
+    b.sync()
+    ref = b.group(0)
+    spans = [b.group(g).profile_spans(ref) for g in range(2)]  # before any profile_kernel resets ref
+    for g in range(2):
+        e = b.group(g)
+        st, en = spans[g]
+        total, n = e.profile_read()
+        assert n == 2 * T and len(st) == n, (g, n, len(st))
+        assert np.all(st >= -1e-3) and np.all(en >= st), g
+        assert np.all(np.diff(st) >= 0) and np.all(st[1:] >= en[:-1] - 1e-3), g  # one stream: in order
+        assert abs(float(np.sum(en - st)) - total) <= 1e-2 * total + 1e-3, (g, float(np.sum(en - st)), total)
+    for g in range(2):
+        b.group(g).profile_kernel("")

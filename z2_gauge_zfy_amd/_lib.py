@@ -125,6 +125,8 @@ _SIGS = {
     "z2_profile_kernel": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_profile_read": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "z2_profile_work": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "z2_profile_spans": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int64,
+                                   C.POINTER(C.c_int64)]),
     "z2_set_kin_tiles": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "z2_set_u_precision": (C.c_int, [C.c_void_p, C.c_int32]),
     "z2_batch_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,

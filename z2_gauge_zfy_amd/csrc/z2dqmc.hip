@@ -162,6 +162,7 @@ struct z2_ctx {
   int left_done = 0;
   // kernel timing hooks: HIP events around every launch of one kernel class
   int prof_kind = 0;
+  hipEvent_t prof_ref = nullptr;  // recorded by z2_profile_kernel: time zero of z2_profile_spans
   std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev;
   size_t prof_used = 0;
   double prof_work = 0.0;
@@ -203,6 +204,7 @@ static void free_ctx(z2_ctx *c) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
   }
+  if (c->prof_ref) (void)hipEventDestroy(c->prof_ref);
   for (void *p : c->allocs) (void)hipFree(p);
   c->allocs.clear();
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -3036,9 +3038,29 @@ extern "C" int z2_test_trsm(int32_t n, const double *R, double *X, int32_t right
 extern "C" int z2_profile_kernel(z2_ctx *c, int32_t kind) {
   if (!c || kind < 0 || kind > Z2_K_QT_APPLY) return set_err(Z2_EINVAL, "bad kernel class %d", kind);
   HIPC(hipStreamSynchronize(c->stream));
+  if (!c->prof_ref) HIPC(hipEventCreate(&c->prof_ref));
+  HIPC(hipEventRecord(c->prof_ref, c->stream));
   c->prof_kind = kind;
   c->prof_used = 0;
   c->prof_work = 0.0;
+  return Z2_OK;
+}
+
+extern "C" int z2_profile_spans(z2_ctx *c, z2_ctx *ref, double *start_ms, double *end_ms, int64_t cap,
+                                int64_t *n) {
+  if (!c || !ref || !start_ms || !end_ms || !n || cap < 0) return set_err(Z2_EINVAL, "bad argument");
+  if (!ref->prof_ref) return set_err(Z2_ESTATE, "z2_profile_kernel has not run on the reference context");
+  HIPC(hipStreamSynchronize(c->stream));
+  HIPC(hipStreamSynchronize(ref->stream));
+  const int64_t m = std::min<int64_t>((int64_t)c->prof_used, cap);
+  for (int64_t k = 0; k < m; ++k) {
+    float a = 0.f, b = 0.f;
+    HIPC(hipEventElapsedTime(&a, ref->prof_ref, c->prof_ev[k].first));
+    HIPC(hipEventElapsedTime(&b, ref->prof_ref, c->prof_ev[k].second));
+    start_ms[k] = a;
+    end_ms[k] = b;
+  }
+  *n = m;
   return Z2_OK;
 }
 

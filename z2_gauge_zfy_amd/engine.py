@@ -290,6 +290,13 @@ class Engine:
         check(lib().z2_profile_read(self._ptr, C.byref(t), C.byref(n)))
         return t.value, n.value
 
+    def profile_spans(self, ref=None, cap: int = 1 << 20):
+        """(start, end) in ms of the timed launches since ref's profile_kernel (ref: an
+        engine view on the same device, profiled first; default self)"""
+        st, en, n = np.zeros(cap), np.zeros(cap), C.c_int64()
+        check(lib().z2_profile_spans(self._ptr, (ref or self).ptr, _pd(st), _pd(en), cap, C.byref(n)))
+        return st[:n.value], en[:n.value]
+
     def profile_work(self) -> float:
         """algorithmic flops (or bytes) of the launches timed since profile_kernel"""
         w = C.c_double()
