@@ -37,7 +37,6 @@ __host__ __device__ inline size_t flip_rg_lds_bytes(int N) {
   const size_t N2 = 2 * (size_t)N;
   size_t b = sizeof(double) * (4 * 4 * (size_t)N + 4 * N2 + 16);  // 4 update slots, 2x2 per bond, boson
   b += sizeof(float) * (2 + 8) * (size_t)N;  // columns p1, p2; rows p1, p2 as float4 per column
-  b += sizeof(short) * 2 * N2;               // bond sites per bond
   b += N2;                                   // current field
   return (b + 15) & ~(size_t)15;
 }
@@ -107,8 +106,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   float *col = reinterpret_cast<float *>(bos + 16);    // 2 x N : G(:,p1), G(:,p2)
   float *row4 = col + 2 * N;                           // 2 x N x 4: G(p,:) as the float4 of rows
                                                        //   4(p/4) .. 4(p/4)+3 per column
-  short *bp = reinterpret_cast<short *>(row4 + 8 * N); // 2N x 2: bond sites
-  int8_t *sg = reinterpret_cast<int8_t *>(bp + 2 * N2);   // 2N : current slice field
+  int8_t *sg = reinterpret_cast<int8_t *>(row4 + 8 * N);  // 2N : current slice field
   __shared__ int s_kmin[2];
   __shared__ uint32_t s_dec[64];
 
@@ -148,16 +146,16 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       g[tr][tc] = x;
     }
   // lattice tables and the field of the slice into LDS
-  for (int b = tid; b < N2; b += nth) {
-    bp[2 * b] = (short)a.lat.bond_p[2 * b];
-    bp[2 * b + 1] = (short)a.lat.bond_p[2 * b + 1];
-    sg[b] = sig_t[b];
-  }
+  for (int b = tid; b < N2; b += nth) sg[b] = sig_t[b];
   if (tid < 9) bos[tid] = cp->boson[tid];
   for (int q = tid; q < 64; q += nth) s_dec[q] = 0u;
+  // the decision word: (2 k + (old field > 0)) << 16 | p1 << 8 | p2 for proposal k on the
+  // bond of sites p1, p2 (N = 256: sites fit in 8 bits), so that its minimum over the
+  // accepted proposals is the first one and carries the sites with it
+  constexpr int KEY = 16, NONE = (2 * N2 + 2) << KEY;
   if (tid == 0) {
-    s_kmin[0] = 2 * N2 + 2;
-    s_kmin[1] = 2 * N2 + 2;
+    s_kmin[0] = NONE;
+    s_kmin[1] = NONE;
   }
   const uint32_t *frc = a.force ? a.force + (size_t)chain * a.dwords : nullptr;
   const int gb = a.gbond ? (a.gmask[(size_t)chain * a.T + (a.nt - 1)] ? a.gbond[chain] : -1) : -1;
@@ -171,7 +169,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   while (true) {
     const int slot = round & 1;
     // (E) fold the last accepted update into this thread's undecided proposal, evaluate it
-    int cand = 2 * N2 + 2;
+    int cand = NONE;
     if (k >= start) {
       if (last >= 0) {
         const double *TA = US + (size_t)4 * N * last, *VA = TA + 2 * N;
@@ -197,7 +195,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       const bool acc = a.gbond ? rb_ == gb
                                : (frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rrn);
       if (acc) {
-        cand = 2 * k + (rso > 0 ? 1 : 0);
+        cand = ((2 * k + (rso > 0 ? 1 : 0)) << KEY) | (rp1 << 8) | rp2;
         double *o = o4 + 4 * rb_;
         o[0] = r11;
         o[1] = r21;
@@ -206,14 +204,14 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       }
     }
     cand = wave_min_int(cand);
-    if (lane == 0 && cand < 2 * N2) atomicMin(&s_kmin[slot], cand);
-    if (tid == 0) s_kmin[slot ^ 1] = 2 * N2 + 2;
+    if (lane == 0 && cand < NONE) atomicMin(&s_kmin[slot], cand);
+    if (tid == 0) s_kmin[slot ^ 1] = NONE;
     __syncthreads();
-    const int packed = s_kmin[slot];
-    if (packed >= 2 * N2) break;
-    const int kmin = packed >> 1, s_old = (packed & 1) ? 1 : -1;
+    const int packed = __builtin_amdgcn_readfirstlane(s_kmin[slot]);
+    if (packed >= NONE) break;
+    const int kmin = packed >> (KEY + 1), s_old = ((packed >> KEY) & 1) ? 1 : -1;
     const int b = a.descending ? (N2 - 1 - kmin) : kmin;
-    const int p1 = __builtin_amdgcn_readfirstlane(bp[2 * b]), p2 = __builtin_amdgcn_readfirstlane(bp[2 * b + 1]);
+    const int p1 = (packed >> 8) & 255, p2 = packed & 255;
     // (X) columns p1, p2 and rows p1, p2 of the register G into LDS
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
