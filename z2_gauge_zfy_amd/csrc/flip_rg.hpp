@@ -37,27 +37,29 @@ __host__ __device__ inline size_t flip_rg_lds_bytes(int N) {
   const size_t N2 = 2 * (size_t)N;
   size_t b = sizeof(double) * (4 * 4 * (size_t)N + 4 * N2 + 16);  // 4 update slots, 2x2 per bond, boson
   b += sizeof(float) * (2 + 8) * (size_t)N;  // columns p1, p2; rows p1, p2 as float4 per column
+  b += sizeof(float) * 4 * 4 * (size_t)N;    // the update slots as the products' FP32 operands
   b += N2;                                   // current field
   return (b + 15) & ~(size_t)15;
 }
 
 // the rank-4 (both = true) or rank-2 update of the register tiles:
-// G(r, c) -= sum_k u_k(r) v_k(c) over the vectors of slots sa (and sb)
+// G(r, c) -= sum_k u_k(r) v_k(c) over the vectors of slots sa (and sb), from the
+// slots' FP32 copies (u_c(i) at 2i + c, -v_c(i) at 2N + 2i + c)
 template <int TRW, int TCW>
-__device__ __forceinline__ void rg_update(rg_f4 (&g)[TRW][TCW], const double *US, int N, int R0, int C0, int li,
+__device__ __forceinline__ void rg_update(rg_f4 (&g)[TRW][TCW], const float *UF, int N, int R0, int C0, int li,
                                           int lk, int sa, int sb, bool both) {
   const int cmp = lk & 1;
-  const double *U = US + (size_t)4 * N * (lk < 2 ? sa : sb);
+  const float *U = UF + (size_t)4 * N * (lk < 2 ? sa : sb);
   const bool on = lk < 2 || both;
   float av[TRW], bv[TCW];
 #pragma unroll
   for (int tr = 0; tr < TRW; ++tr) {
-    const float t = (float)U[2 * (R0 + 16 * tr + li) + cmp];
+    const float t = U[2 * (R0 + 16 * tr + li) + cmp];
     av[tr] = on ? t : 0.f;  // u_k(row)
   }
 #pragma unroll
   for (int tc = 0; tc < TCW; ++tc) {
-    const float v = (float)(-U[2 * N + 2 * (C0 + 16 * tc + li) + cmp]);
+    const float v = U[2 * N + 2 * (C0 + 16 * tc + li) + cmp];
     bv[tc] = on ? v : 0.f;  // -v_k(col)
   }
 #pragma unroll
@@ -106,7 +108,8 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   float *col = reinterpret_cast<float *>(bos + 16);    // 2 x N : G(:,p1), G(:,p2)
   float *row4 = col + 2 * N;                           // 2 x N x 4: G(p,:) as the float4 of rows
                                                        //   4(p/4) .. 4(p/4)+3 per column
-  int8_t *sg = reinterpret_cast<int8_t *>(row4 + 8 * N);  // 2N : current slice field
+  float *USF = row4 + 8 * N;                           // 4 x 4N: the slots in FP32, v negated
+  int8_t *sg = reinterpret_cast<int8_t *>(USF + 16 * N);  // 2N : current slice field
   __shared__ int s_kmin[2];
   __shared__ uint32_t s_dec[64];
 
@@ -326,19 +329,24 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
     };
     const int ns = round & 3;
     double *TN = US + (size_t)4 * N * ns;
+    float *TF = USF + (size_t)4 * N * ns;
     if (tid < N) {
       double u0, u1;
       u_at(tid, u0, u1);
       TN[2 * tid] = u0;
       TN[2 * tid + 1] = u1;
+      TF[2 * tid] = (float)u0;
+      TF[2 * tid + 1] = (float)u1;
     } else {
       double v0, v1;
       v_at(tid - N, v0, v1);
       TN[2 * N + 2 * (tid - N)] = v0;
       TN[2 * N + 2 * (tid - N) + 1] = v1;
+      TF[2 * N + 2 * (tid - N)] = (float)(-v0);
+      TF[2 * N + 2 * (tid - N) + 1] = (float)(-v1);
     }
     // the two pending updates into the registers (the extraction above has read them)
-    if (pq1 >= 0) rg_update<TRW, TCW>(g, US, N, R0, C0, li, lk, pq0, pq1, true);
+    if (pq1 >= 0) rg_update<TRW, TCW>(g, USF, N, R0, C0, li, lk, pq0, pq1, true);
     __syncthreads();
     if (pq1 >= 0) {
       pq0 = ns;
@@ -353,7 +361,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
     start = kmin + 1;
     round = round + 1;
   }
-  if (pq0 >= 0) rg_update<TRW, TCW>(g, US, N, R0, C0, li, lk, pq0, pq1 >= 0 ? pq1 : pq0, pq1 >= 0);
+  if (pq0 >= 0) rg_update<TRW, TCW>(g, USF, N, R0, C0, li, lk, pq0, pq1 >= 0 ? pq1 : pq0, pq1 >= 0);
   // G back to memory (right pass: with the mu factors, as k_flip's final flush)
   const float fw1 = a.final_flush == 1 ? (float)ep : 1.f, fw2 = a.final_flush == 1 ? (float)em : 1.f;
 #pragma unroll
