@@ -35,7 +35,7 @@ typedef float rg_f4 __attribute__((ext_vector_type(4)));
 // LDS bytes of k_flip_rg for N sites
 __host__ __device__ inline size_t flip_rg_lds_bytes(int N) {
   const size_t N2 = 2 * (size_t)N;
-  size_t b = sizeof(double) * (4 * 4 * (size_t)N + 4 * N2 + 16);  // 4 update slots, 2x2 per bond, boson
+  size_t b = sizeof(double) * (4 * 4 * (size_t)N + 6 * N2 + 16);  // 4 update slots, Prod^-1 and det per bond, boson
   b += sizeof(float) * (2 + 8) * (size_t)N;  // columns p1, p2; rows p1, p2 as float4 per column
   b += sizeof(float) * 4 * 4 * (size_t)N;    // the update slots as the products' FP32 operands
   b += N2;                                   // current field
@@ -103,15 +103,14 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   extern __shared__ unsigned char smem_raw[];
   double *US = reinterpret_cast<double *>(smem_raw);   // 4 x 4N: update slots, u_c(i) at 2i + c,
                                                        //         v_c(i) at 2N + 2i + c
-  double *o4 = US + (size_t)16 * N;                    // 2N x 4: 2x2 block of an accepted candidate
-  double *bos = o4 + (size_t)4 * N2;                   // 9 (16): boson ratio table
+  double *o4 = US + (size_t)16 * N;                    // 2N x 6: Prod^{-1} (4) and det of an accepted candidate
+  double *bos = o4 + (size_t)6 * N2;                   // 9 (16): boson ratio table
   float *col = reinterpret_cast<float *>(bos + 16);    // 2 x N : G(:,p1), G(:,p2)
   float *row4 = col + 2 * N;                           // 2 x N x 4: G(p,:) as the float4 of rows
                                                        //   4(p/4) .. 4(p/4)+3 per column
   float *USF = row4 + 8 * N;                           // 4 x 4N: the slots in FP32, v negated
   int8_t *sg = reinterpret_cast<int8_t *>(USF + 16 * N);  // 2N : current slice field
   __shared__ int s_kmin[2];
-  __shared__ uint32_t s_dec[64];
 
   const double Dp0 = cp->delta[0][0], Dp1 = cp->delta[0][1], Dp2 = cp->delta[0][2], Dp3 = cp->delta[0][3];
   const double Dm0 = cp->delta[1][0], Dm1 = cp->delta[1][1], Dm2 = cp->delta[1][2], Dm3 = cp->delta[1][3];
@@ -151,7 +150,6 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   // lattice tables and the field of the slice into LDS
   for (int b = tid; b < N2; b += nth) sg[b] = sig_t[b];
   if (tid < 9) bos[tid] = cp->boson[tid];
-  for (int q = tid; q < 64; q += nth) s_dec[q] = 0u;
   // the decision word: (2 k + (old field > 0)) << 16 | p1 << 8 | p2 for proposal k on the
   // bond of sites p1, p2 (N = 256: sites fit in 8 bits), so that its minimum over the
   // accepted proposals is the first one and carries the sites with it
@@ -168,6 +166,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
   const double D2 = rso > 0 ? Dp2 : Dm2, D3 = rso > 0 ? Dp3 : Dm3;
 
   int start = 0, round = 0, tot = 0, last = -1, lastb = -1;
+  bool mine = false;  // this thread's proposal was accepted
   int pq0 = -1, pq1 = -1;  // slots of the accepted updates not yet in the registers, in order
   while (true) {
     const int slot = round & 1;
@@ -198,12 +197,15 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       const bool acc = a.gbond ? rb_ == gb
                                : (frc ? ((frc[k >> 5] >> (k & 31)) & 1u) != 0u : fabs(ratio) > rrn);
       if (acc) {
+        // Prod^{-1} for the update, should this proposal be the first accepted one
         cand = ((2 * k + (rso > 0 ? 1 : 0)) << KEY) | (rp1 << 8) | rp2;
-        double *o = o4 + 4 * rb_;
-        o[0] = r11;
-        o[1] = r21;
-        o[2] = r12;
-        o[3] = r22;
+        const double rdet = 1.0 / det;
+        double *o = o4 + 6 * rb_;
+        o[0] = P22 * rdet;
+        o[1] = (-P21) * rdet;
+        o[2] = (-P12) * rdet;
+        o[3] = P11 * rdet;
+        o[4] = det;
       }
     }
     cand = wave_min_int(cand);
@@ -215,6 +217,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
     const int kmin = packed >> (KEY + 1), s_old = ((packed >> KEY) & 1) ? 1 : -1;
     const int b = a.descending ? (N2 - 1 - kmin) : kmin;
     const int p1 = (packed >> 8) & 255, p2 = packed & 255;
+    mine = mine || k == kmin;
     // (X) columns p1, p2 and rows p1, p2 of the register G into LDS
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -248,32 +251,18 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
         }
       }
     }
-    // the accepted proposal's 2x2 block as its thread evaluated it, and Prod^{-1}
+    // Prod^{-1} of the accepted proposal as its thread computed it when evaluating it
     const double D11 = s_old > 0 ? Dp0 : Dm0, D21 = s_old > 0 ? Dp1 : Dm1;
     const double D12 = s_old > 0 ? Dp2 : Dm2, D22 = s_old > 0 ? Dp3 : Dm3;
-    double Pi11, Pi21, Pi12, Pi22;
-    {
-      const double *o = o4 + 4 * b;
-      const double A11 = 1.0 - o[0], A21 = 0.0 - o[1];
-      const double A12 = 0.0 - o[2], A22 = 1.0 - o[3];
-      const double P11 = 1.0 + (D11 * A11 + D12 * A21);
-      const double P21 = 0.0 + (D21 * A11 + D22 * A21);
-      const double P12 = 0.0 + (D11 * A12 + D12 * A22);
-      const double P22 = 1.0 + (D21 * A12 + D22 * A22);
-      const double det = P11 * P22 - P12 * P21;
-      const double rdet = 1.0 / det;
-      Pi11 = P22 * rdet;
-      Pi21 = (-P21) * rdet;
-      Pi12 = (-P12) * rdet;
-      Pi22 = P11 * rdet;
-      if (a.gbond && tid == 0) {
-        double r = fabs(det * det);
-        a.glr[2 * chain] = a.glr[2 * chain] + log(r > 1.0e-300 ? r : 1.0e-300);
-      }
+    const double *ob = o4 + 6 * b;
+    const double Pi11 = ob[0], Pi21 = ob[1], Pi12 = ob[2], Pi22 = ob[3];
+    if (a.gbond && tid == 0) {
+      const double det = ob[4];
+      double r = fabs(det * det);
+      a.glr[2 * chain] = a.glr[2 * chain] + log(r > 1.0e-300 ? r : 1.0e-300);
     }
     if (tid == 0) {
       sg[b] = (int8_t)(-s_old);
-      s_dec[kmin >> 5] |= 1u << (kmin & 31);
       tot = tot + 1;
     }
     __syncthreads();
@@ -330,6 +319,12 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
     const int ns = round & 3;
     double *TN = US + (size_t)4 * N * ns;
     float *TF = USF + (size_t)4 * N * ns;
+    // the two pending updates into the registers (the extraction above has read them):
+    // half of the wavefronts issue their products before this phase's vector work, half
+    // after it, so that each SIMD's matrix core starts on one wavefront's products while
+    // its other wavefront computes (early: w and w ^ 1, and w and w ^ 4, differ)
+    const bool early = ((wave ^ (wave >> 2)) & 1) == 0;
+    if (pq1 >= 0 && early) rg_update<TRW, TCW>(g, USF, N, R0, C0, li, lk, pq0, pq1, true);
     if (tid < N) {
       double u0, u1;
       u_at(tid, u0, u1);
@@ -345,8 +340,7 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       TF[2 * N + 2 * (tid - N)] = (float)(-v0);
       TF[2 * N + 2 * (tid - N) + 1] = (float)(-v1);
     }
-    // the two pending updates into the registers (the extraction above has read them)
-    if (pq1 >= 0) rg_update<TRW, TCW>(g, USF, N, R0, C0, li, lk, pq0, pq1, true);
+    if (pq1 >= 0 && !early) rg_update<TRW, TCW>(g, USF, N, R0, C0, li, lk, pq0, pq1, true);
     __syncthreads();
     if (pq1 >= 0) {
       pq0 = ns;
@@ -377,8 +371,13 @@ __global__ void __launch_bounds__(512) k_flip_rg(FlipArgs<float> a) {
       *reinterpret_cast<rg_f4 *>(G + R0 + 16 * tr + 4 * lk + (size_t)(C0 + 16 * tc + li) * N) = x;
     }
   for (int bb = tid; bb < N2; bb += nth) sig_t[bb] = sg[bb];
-  if (a.record)
-    for (int q = tid; q < a.dwords; q += nth) a.record[(size_t)chain * a.dwords + q] = s_dec[q];
+  if (a.record) {  // proposal k = tid: bit k of the decision words, one 64-bit ballot per wavefront
+    const unsigned long long bits = __ballot(mine);
+    if (lane == 0) {
+      a.record[(size_t)chain * a.dwords + 2 * wave] = (uint32_t)bits;
+      a.record[(size_t)chain * a.dwords + 2 * wave + 1] = (uint32_t)(bits >> 32);
+    }
+  }
   if (tid == 0) {
     a.nacc[chain] = 0;
     a.mu_done[chain] = mu;
