@@ -45,7 +45,7 @@ def _c3():
 
 
 CONFIGS = {
-    "c4": dict(chains=_grid(16, 200, 10, 20.0, 0.0, (0.4, 0.6, 0.8, 1.0), 64), precision="both", groups=2,
+    "c4": dict(chains=_grid(16, 200, 10, 20.0, 0.0, (0.4, 0.6, 0.8, 1.0), 64), precision="both", groups=3,
                extra_groups=1,
                desc="64 chains, L=16 (N=256), beta=20, Ltrot=200, Nwrap=10, h in [0.4, 0.6, 0.8, 1.0] x 16, "
                     "J=1, t=1, mu=0"),
