@@ -105,12 +105,20 @@ __device__ __forceinline__ R kin_epilogue(double x, const KinMat<R> &km, const C
 // Rotation table of one slice in LDS: for each of the 4 checkerboard groups and
 // each of its N/2 bonds, the two sites and the sign of nflag * sigma packed in
 // one int (sites < 2048), so a rotation needs one LDS read instead of a chain of
-// dependent global reads.
-__device__ __forceinline__ int kin_tab_entry(const LatticeDev &lat, const int8_t *s, int g, int k, int nflag) {
-  const int half = lat.N / 2;
-  const int b = lat.grp[g * half + k];
-  const int p1 = lat.bond_p[2 * b], p2 = lat.bond_p[2 * b + 1];
-  return p1 | (p2 << 11) | ((nflag * (int)s[b] > 0) ? (1 << 22) : 0);
+// dependent global reads.  Entry e = g N/2 + k (bond k of group g); the block's
+// nt threads build it two entries per thread per step, both entries' dependent
+// reads in flight together (the second index clamped to the last entry: past
+// the end it rewrites that entry with its own value).
+__device__ __forceinline__ void kin_tab_build(int *tab, const LatticeDev &lat, const int8_t *s, int nflag, int n,
+                                              int tid, int nt) {
+  for (int e0 = tid; e0 < n; e0 += 2 * nt) {
+    const int e1 = min(e0 + nt, n - 1);
+    const int b0 = lat.grp[e0], b1 = lat.grp[e1];
+    const int p10 = lat.bond_p[2 * b0], p20 = lat.bond_p[2 * b0 + 1], s0 = s[b0];
+    const int p11 = lat.bond_p[2 * b1], p21 = lat.bond_p[2 * b1 + 1], s1 = s[b1];
+    tab[e0] = p10 | (p20 << 11) | ((nflag * s0 > 0) ? (1 << 22) : 0);
+    tab[e1] = p11 | (p21 << 11) | ((nflag * s1 > 0) ? (1 << 22) : 0);
+  }
 }
 
 // Mat <- B^{nflag} Mat : each column is independent; a block owns CB columns.
@@ -132,8 +140,7 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
   const ChainConst c = a.cc[chain];
   if (km.nflag != 0) {
     const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
-    for (int g = 0; g < 4; ++g)
-      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, s, g, k, km.nflag);
+    kin_tab_build(tab, a.lat, s, km.nflag, 4 * half, tid, nt);
   }
   for (int cc = 0; cc < ncol; ++cc)
     for (int r = tid; r < N; r += nt) tile[cc * N + r] = kin_prologue<R>(M[r + (size_t)(col0 + cc) * N], r, col0 + cc, km, a, c, chain);
@@ -181,8 +188,7 @@ __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
   const int r = tid % RB, cstep = nt / RB;
   if (km.nflag != 0) {
     const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
-    for (int g = 0; g < 4; ++g)
-      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, s, g, k, km.nflag);
+    kin_tab_build(tab, a.lat, s, km.nflag, 4 * half, tid, nt);
   }
   if (r < nrow)
     for (int col = tid / RB; col < N; col += cstep)
@@ -257,8 +263,7 @@ __global__ void __launch_bounds__(256) k_kin_rows_multi(KinArgs<R> a, int nt0, i
   const int items = half * ngrp;
   for (int s = 0; s < ns; ++s) {
     const int8_t *sg = a.sig + (size_t)chain * a.T * N2 + (size_t)(nt0 + s * step - 1) * N2;
-    for (int g = 0; g < 4; ++g)
-      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, sg, g, k, km.nflag);
+    kin_tab_build(tab, a.lat, sg, km.nflag, 4 * half, tid, nt);
     for (int cc = 0; cc < ncol; ++cc)
       for (int r = tid; r < N; r += nt) tile[cc * N + r] = kin_multi_pre<R>(tile[cc * N + r], km, c);
     __syncthreads();
@@ -304,8 +309,7 @@ __global__ void __launch_bounds__(256) k_kin_cols_multi(KinArgs<R> a, int nt0, i
   const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
   for (int s = 0; s < ns; ++s) {
     const int8_t *sg = a.sig + (size_t)chain * a.T * N2 + (size_t)(nt0 + s * step - 1) * N2;
-    for (int g = 0; g < 4; ++g)
-      for (int k = tid; k < half; k += nt) tab[g * half + k] = kin_tab_entry(a.lat, sg, g, k, km.nflag);
+    kin_tab_build(tab, a.lat, sg, km.nflag, 4 * half, tid, nt);
     if (r < nrow)
       for (int col = tid / RB; col < N; col += cstep) tile[col * RB + r] = kin_multi_pre<R>(tile[col * RB + r], km, c);
     __syncthreads();
