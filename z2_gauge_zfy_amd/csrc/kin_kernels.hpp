@@ -63,9 +63,9 @@ __device__ __forceinline__ double apply_fac(double x, int sel, const ChainConst 
 // rounded result.
 template <typename R>
 __device__ __forceinline__ double kin_prologue(R xin, int r, int col, const KinMat<R> &km,
-                                               const KinArgs<R> &a, const ChainConst &c, int chain) {
+                                               const KinArgs<R> &a, const ChainConst &c, int chain, bool pre_on) {
   double x = (double)xin;
-  if (!(km.pre_cond && a.mu_done[chain])) {
+  if (pre_on) {
     x = apply_fac<R>(x, km.pre1, c);
     x = apply_fac<R>(x, km.pre2, c);
   }
@@ -101,6 +101,9 @@ __device__ __forceinline__ R kin_epilogue(double x, const KinMat<R> &km, const C
   }
   return (R)x;
 }
+
+// threads per block of every kinetic launch (the kernels' launch bounds)
+constexpr int KIN_NT = 256;
 
 // Rotation table of one slice in LDS: for each of the 4 checkerboard groups and
 // each of its N/2 bonds, the two sites and the sign of nflag * sigma packed in
@@ -142,8 +145,29 @@ __global__ void __launch_bounds__(256) k_kin_rows(KinArgs<R> a) {
     const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
     kin_tab_build(tab, a.lat, s, km.nflag, 4 * half, tid, nt);
   }
-  for (int cc = 0; cc < ncol; ++cc)
-    for (int r = tid; r < N; r += nt) tile[cc * N + r] = kin_prologue<R>(M[r + (size_t)(col0 + cc) * N], r, col0 + cc, km, a, c, chain);
+  // the tile, the reads of a thread all in flight before the prologue (sizes fixed
+  // at compile time: the block has KIN_NT threads)
+  const bool pre_on = !(km.pre_cond && a.mu_done[chain]);
+  if constexpr (NC > 0) {
+    constexpr int RPT = (NC + KIN_NT - 1) / KIN_NT;
+    R raw[CB][RPT];
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc)
+#pragma unroll
+      for (int j = 0; j < RPT; ++j)
+        raw[cc][j] = M[min(tid + j * KIN_NT, NC - 1) + (size_t)(col0 + min(cc, ncol - 1)) * NC];
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc)
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * KIN_NT;
+        if (cc < ncol && r < NC) tile[cc * NC + r] = kin_prologue<R>(raw[cc][j], r, col0 + cc, km, a, c, chain, pre_on);
+      }
+  } else {
+    for (int cc = 0; cc < ncol; ++cc)
+      for (int r = tid; r < N; r += nt)
+        tile[cc * N + r] = kin_prologue<R>(M[r + (size_t)(col0 + cc) * N], r, col0 + cc, km, a, c, chain, pre_on);
+  }
   __syncthreads();
   if (km.nflag != 0) {
     const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
@@ -190,9 +214,24 @@ __global__ void __launch_bounds__(256) k_kin_cols(KinArgs<R> a) {
     const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
     kin_tab_build(tab, a.lat, s, km.nflag, 4 * half, tid, nt);
   }
-  if (r < nrow)
+  // the tile, the reads of a thread all in flight before the prologue
+  const bool pre_on = !(km.pre_cond && a.mu_done[chain]);
+  if constexpr (NC > 0) {
+    constexpr int CSTEP = KIN_NT / RB, CPT = (NC + CSTEP - 1) / CSTEP;
+    R raw[CPT];
+    const int rr = row0 + min(r, nrow - 1);
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) raw[j] = M[rr + (size_t)min(tid / RB + j * CSTEP, NC - 1) * NC];
+    if (r < nrow)
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const int col = tid / RB + j * CSTEP;
+        if (col < NC) tile[col * RB + r] = kin_prologue<R>(raw[j], row0 + r, col, km, a, c, chain, pre_on);
+      }
+  } else if (r < nrow) {
     for (int col = tid / RB; col < N; col += cstep)
-      tile[col * RB + r] = kin_prologue<R>(M[(row0 + r) + (size_t)col * N], row0 + r, col, km, a, c, chain);
+      tile[col * RB + r] = kin_prologue<R>(M[(row0 + r) + (size_t)col * N], row0 + r, col, km, a, c, chain, pre_on);
+  }
   __syncthreads();
   if (km.nflag != 0) {
     const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
@@ -228,7 +267,7 @@ __global__ void __launch_bounds__(256) k_kin_elem(KinArgs<R> a) {
   for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < N * N;
        idx += gridDim.x * blockDim.x) {
     int col = idx / N, r = idx - col * N;
-    M[idx] = kin_epilogue<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain), km, c);
+    M[idx] = kin_epilogue<R>(kin_prologue<R>(M[idx], r, col, km, a, c, chain, !(km.pre_cond && a.mu_done[chain])), km, c);
   }
 }
 
@@ -256,8 +295,25 @@ __global__ void __launch_bounds__(256) k_kin_rows_multi(KinArgs<R> a, int nt0, i
   const int col0 = blockIdx.x * CB;
   const int ncol = min(CB, N - col0);
   const ChainConst c = a.cc[chain];
-  for (int cc = 0; cc < ncol; ++cc)
-    for (int r = tid; r < N; r += nt) tile[cc * N + r] = (double)M[r + (size_t)(col0 + cc) * N];
+  if constexpr (NC > 0) {  // the reads of a thread all in flight
+    constexpr int RPT = (NC + KIN_NT - 1) / KIN_NT;
+    R raw[CB][RPT];
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc)
+#pragma unroll
+      for (int j = 0; j < RPT; ++j)
+        raw[cc][j] = M[min(tid + j * KIN_NT, NC - 1) + (size_t)(col0 + min(cc, ncol - 1)) * NC];
+#pragma unroll
+    for (int cc = 0; cc < CB; ++cc)
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int r = tid + j * KIN_NT;
+        if (cc < ncol && r < NC) tile[cc * NC + r] = (double)raw[cc][j];
+      }
+  } else {
+    for (int cc = 0; cc < ncol; ++cc)
+      for (int r = tid; r < N; r += nt) tile[cc * N + r] = (double)M[r + (size_t)(col0 + cc) * N];
+  }
   const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
   const int ngrp = half >= nt ? 1 : nt / half;
   const int items = half * ngrp;
@@ -304,8 +360,21 @@ __global__ void __launch_bounds__(256) k_kin_cols_multi(KinArgs<R> a, int nt0, i
   const int nrow = min(RB, N - row0);
   const ChainConst c = a.cc[chain];
   const int r = tid % RB, cstep = nt / RB;
-  if (r < nrow)
+  if constexpr (NC > 0) {  // the reads of a thread all in flight
+    constexpr int CSTEP = KIN_NT / RB, CPT = (NC + CSTEP - 1) / CSTEP;
+    R raw[CPT];
+    const int rr = row0 + min(r, nrow - 1);
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) raw[j] = M[rr + (size_t)min(tid / RB + j * CSTEP, NC - 1) * NC];
+    if (r < nrow)
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const int col = tid / RB + j * CSTEP;
+        if (col < NC) tile[col * RB + r] = (double)raw[j];
+      }
+  } else if (r < nrow) {
     for (int col = tid / RB; col < N; col += cstep) tile[col * RB + r] = (double)M[(row0 + r) + (size_t)col * N];
+  }
   const double Cp = c.Cp, Cm = c.Cm, Sp = c.Sp, Sm = c.Sm;
   for (int s = 0; s < ns; ++s) {
     const int8_t *sg = a.sig + (size_t)chain * a.T * N2 + (size_t)(nt0 + s * step - 1) * N2;

@@ -983,7 +983,7 @@ static int launch_rows_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
   size_t sh = sizeof(double) * CB * c->N + sizeof(int) * 2 * c->N;  // FP64 tile + rotation table
   if (sh > 65536)
     HIPC(hipFuncSetAttribute((const void *)k_kin_rows<R, CB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  hipLaunchKernelGGL((k_kin_rows<R, CB, NC>), grid, dim3(256), sh, c->stream, a);
+  hipLaunchKernelGGL((k_kin_rows<R, CB, NC>), grid, dim3(KIN_NT), sh, c->stream, a);
   LAUNCH_CHECK();
   return Z2_OK;
 }
@@ -994,7 +994,7 @@ static int launch_cols_t(z2_ctx *c, KinArgs<R> &a, int nmat) {
   size_t sh = sizeof(double) * RB * c->N + sizeof(int) * 2 * c->N;  // FP64 tile + rotation table
   if (sh > 65536)
     HIPC(hipFuncSetAttribute((const void *)k_kin_cols<R, RB, NC>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  hipLaunchKernelGGL((k_kin_cols<R, RB, NC>), grid, dim3(256), sh, c->stream, a);
+  hipLaunchKernelGGL((k_kin_cols<R, RB, NC>), grid, dim3(KIN_NT), sh, c->stream, a);
   LAUNCH_CHECK();
   return Z2_OK;
 }
@@ -1759,16 +1759,16 @@ static int launch_u_multi_n(z2_ctx *c, KinArgs<R> &a, int cols, int nt0, int ns,
     const void *fn = RB == 32 ? (const void *)k_kin_cols_multi<R, 32, NC> : (const void *)k_kin_cols_multi<R, 8, NC>;
     if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     dim3 grid((c->N + RB - 1) / RB, c->nch, 1);
-    if (RB == 32) hipLaunchKernelGGL((k_kin_cols_multi<R, 32, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
-    else hipLaunchKernelGGL((k_kin_cols_multi<R, 8, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
+    if (RB == 32) hipLaunchKernelGGL((k_kin_cols_multi<R, 32, NC>), grid, dim3(KIN_NT), sh, c->stream, a, nt0, ns, step);
+    else hipLaunchKernelGGL((k_kin_cols_multi<R, 8, NC>), grid, dim3(KIN_NT), sh, c->stream, a, nt0, ns, step);
   } else {
     const int CB = c->kin_cb;
     const size_t sh = sizeof(double) * CB * c->N + tab;
     const void *fn = CB == 16 ? (const void *)k_kin_rows_multi<R, 16, NC> : (const void *)k_kin_rows_multi<R, 8, NC>;
     if (sh > 65536) HIPC(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     dim3 grid((c->N + CB - 1) / CB, c->nch, 1);
-    if (CB == 16) hipLaunchKernelGGL((k_kin_rows_multi<R, 16, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
-    else hipLaunchKernelGGL((k_kin_rows_multi<R, 8, NC>), grid, dim3(256), sh, c->stream, a, nt0, ns, step);
+    if (CB == 16) hipLaunchKernelGGL((k_kin_rows_multi<R, 16, NC>), grid, dim3(KIN_NT), sh, c->stream, a, nt0, ns, step);
+    else hipLaunchKernelGGL((k_kin_rows_multi<R, 8, NC>), grid, dim3(KIN_NT), sh, c->stream, a, nt0, ns, step);
   }
   LAUNCH_CHECK();
   return Z2_OK;
