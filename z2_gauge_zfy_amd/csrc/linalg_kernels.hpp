@@ -623,6 +623,36 @@ __global__ void __launch_bounds__(256) k_trsm(const double *Rm, double *X, int N
 // descending k with one multiply and one subtract each: the arithmetic and its
 // order are those of k_trsm (and ZTRSM).  Columns of R are staged in LDS 16 at
 // a time.  grid: (ceil(N/16), chains), 256 threads = 16 columns.
+// lane kk of every 16-lane row of the wavefront, in all lanes of that row (DPP
+// row_newbcast; kk a constant once the caller's loop is unrolled)
+template <int KK>
+__device__ __forceinline__ double trsm_row_bcast_t(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x150 + KK, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x150 + KK, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double trsm_row_bcast(double v, int kk) {
+  switch (kk) {
+    case 0: return trsm_row_bcast_t<0>(v);
+    case 1: return trsm_row_bcast_t<1>(v);
+    case 2: return trsm_row_bcast_t<2>(v);
+    case 3: return trsm_row_bcast_t<3>(v);
+    case 4: return trsm_row_bcast_t<4>(v);
+    case 5: return trsm_row_bcast_t<5>(v);
+    case 6: return trsm_row_bcast_t<6>(v);
+    case 7: return trsm_row_bcast_t<7>(v);
+    case 8: return trsm_row_bcast_t<8>(v);
+    case 9: return trsm_row_bcast_t<9>(v);
+    case 10: return trsm_row_bcast_t<10>(v);
+    case 11: return trsm_row_bcast_t<11>(v);
+    case 12: return trsm_row_bcast_t<12>(v);
+    case 13: return trsm_row_bcast_t<13>(v);
+    case 14: return trsm_row_bcast_t<14>(v);
+    default: return trsm_row_bcast_t<15>(v);
+  }
+}
+
 template <int MJ>
 __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, int N, int rs, int cs, int mode) {
   extern __shared__ unsigned char smem_raw[];
@@ -630,7 +660,7 @@ __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, i
   const int chain = blockIdx.y;
   const double *R = Rm + (size_t)chain * N * N;
   double *Xc = X + (size_t)chain * N * N;
-  const int lane = threadIdx.x & 63, l16 = lane & 15, gbase = lane & ~15;
+  const int lane = threadIdx.x & 63, l16 = lane & 15;
   const int c = blockIdx.x * 16 + (threadIdx.x >> 4);
   const bool cin = c < N;
   double xr[MJ];
@@ -639,16 +669,35 @@ __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, i
     const int i = l16 + 16 * j;
     xr[j] = (cin && i < N) ? Xc[(size_t)i * rs + (size_t)c * cs] : 0.0;
   }
+  // N <= 256 (one row of R per thread): the next block's 16 columns are read into
+  // registers while this block is solved, and stored to LDS at the next step
+  constexpr bool PREF = MJ <= 16;
+  double vn[16];
+  auto read_tile = [&](int jk) {  // all 16 reads in flight (clamped row and column)
+    const int i = min((int)threadIdx.x, N - 1);
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) vn[kk] = R[i + (size_t)min(16 * jk + kk, N - 1) * N];
+  };
+  if constexpr (PREF) read_tile((N - 1) / 16);
   for (int jk = (N - 1) / 16; jk >= 0; --jk) {
     __syncthreads();
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-      double v[16];  // all 16 reads in flight before any is used (clamped column, zeroed below)
+    if constexpr (PREF) {
+      if ((int)threadIdx.x < N)
 #pragma unroll
-      for (int kk = 0; kk < 16; ++kk) v[kk] = R[i + (size_t)min(16 * jk + kk, N - 1) * N];
+        for (int kk = 0; kk < 16; ++kk) rt[kk * N + threadIdx.x] = 16 * jk + kk < N ? vn[kk] : 0.0;
+    } else {
+      for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        double v[16];  // all 16 reads in flight before any is used (clamped column, zeroed below)
 #pragma unroll
-      for (int kk = 0; kk < 16; ++kk) rt[kk * N + i] = 16 * jk + kk < N ? v[kk] : 0.0;
+        for (int kk = 0; kk < 16; ++kk) v[kk] = R[i + (size_t)min(16 * jk + kk, N - 1) * N];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) rt[kk * N + i] = 16 * jk + kk < N ? v[kk] : 0.0;
+      }
     }
     __syncthreads();
+    if constexpr (PREF) {
+      if (jk > 0) read_tile(jk - 1);
+    }
     // the block's rows (slot jk) in one register
     double xb = 0.0;
 #pragma unroll
@@ -667,7 +716,7 @@ __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, i
       const double rkk = rk[k];
       const double xo = mode == 0 ? xb / rkk : (1.0 / rkk) * xb;  // meaningful in lane kk
       if (l16 == kk) xb = xo;
-      const double xk = __shfl(xo, gbase | kk, 64);
+      const double xk = trsm_row_bcast(xo, kk);  // lane kk's solved value to its 16-lane row
       pv[kk] = xk;
       if (l16 < kk) {
         const double r = rk[ib];
