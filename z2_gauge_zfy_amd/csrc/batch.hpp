@@ -88,7 +88,10 @@ extern "C" int z2_batch_create(const z2_batch_chain *chains, int32_t n, int32_t 
   b->Nsweep = Nsweep;
   b->max_group = max_group;
   // groups of one shape, in order of first appearance; split into near-equal parts
-  // of at most max_group chains
+  // of at most max_group chains, dealt round-robin (chain j of the shape to part
+  // j mod parts), so that every part gets a share of each run of similar chains
+  // (chains next to each other often share couplings, and a launch lasts as long
+  // as its slowest chain)
   std::vector<std::vector<int>> shapes;
   for (int k = 0; k < n; ++k) {
     const z2_batch_chain &c = chains[k];
@@ -106,8 +109,9 @@ extern "C" int z2_batch_create(const z2_batch_chain *chains, int32_t n, int32_t 
   for (auto &g : shapes) {
     const int parts = max_group > 0 ? ((int)g.size() + max_group - 1) / max_group : 1;
     for (int q = 0; q < parts; ++q) {
-      const size_t lo = g.size() * q / parts, hi = g.size() * (q + 1) / parts;
-      b->members.emplace_back(g.begin() + lo, g.begin() + hi);
+      std::vector<int> part;
+      for (size_t j = q; j < g.size(); j += parts) part.push_back(g[j]);
+      b->members.push_back(part);
     }
   }
   b->where.resize(n);
