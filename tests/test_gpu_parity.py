@@ -70,6 +70,16 @@ def test_parity_c1_single_chain():
     _run_parity(4, 40, 4.0, 10, [z.ChainSpec(h=0.5, Beta=4.0, seed=12345)], nsweeps=5)
 
 
+def test_parity_observables_past_the_slot_count():
+    """Nwrap = 20, more slices per stabilization interval than the 16 whose
+    observable sums wait for one k_obs_finish launch: the 17th slice of an
+    interval first adds the 16 pending ones.  The bins must still be the
+    oracle's (flux exact, the rest within 1e-10)."""
+    import z2_gauge_zfy_amd as z
+    specs = [z.ChainSpec(h=0.5 + 0.2 * k, Beta=4.0, seed=4711 + 2 * 7919 * k) for k in range(2)]
+    _run_parity(4, 40, 4.0, 20, specs, nsweeps=2)
+
+
 def test_parity_batched_uneven_couplings_and_mu():
     import z2_gauge_zfy_amd as z
     specs = [z.ChainSpec(h=0.3 + 0.2 * k, mu=(0.3 if k % 2 else 0.0), J=1.0 + 0.1 * k,

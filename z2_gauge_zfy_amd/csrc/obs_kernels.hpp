@@ -23,14 +23,23 @@ struct ObsArgs {
   const double *cosT;    // (2Lx-1) x (2Ly-1) table of cos(qshift_x*dx + pi*dy)
   int Lx, Ly;
   LatticeDev lat;
-  double *part;          // per chain part_stride x 3 tile partial sums
+  double *part;          // per chain and slot: part_stride x 3 tile partial sums
   int part_stride;
+  double *diag;          // per chain and slot: 1 - G(i, i), N values
+  double *flux;          // per chain and slot: the plaquette products of the slice, N values
+  int slot;              // k_obs_tiles: the slot this slice's values go to
 };
+
+// slots per chain: the slices whose sums wait for k_obs_finish
+constexpr int OBS_SLOTS = 16;
 
 // The N x N pair sums (pair_sc_q0, cdw_pi, cdw_shift) of one slice, one 32 x 32
 // tile of (i, j) pairs per workgroup: G(i, j) and G(j, i) of the tile are staged
-// in LDS by coalesced column reads; the tile's three partial sums go to `part`
-// and k_obs_finish adds them by a fixed tree (deterministic).
+// in LDS by coalesced column reads; the tile's three partial sums go to slot
+// a.slot of `part`, and k_obs_finish adds them by a fixed tree (deterministic).
+// The diagonal tiles also store 1 - G(i, i) of their rows and the first tile the
+// slice's plaquette products, so that nothing k_obs_finish reads changes before
+// it runs (the flips that follow change G and the slice's field).
 #define OBS_TB 32
 template <typename R>
 __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
@@ -50,12 +59,21 @@ __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
     A[c][r] = (ia < N && ja < N) ? (double)G[ia + (size_t)ja * N] : 0.0;
     B[c][r] = (ib < N && jb < N) ? (double)G[ib + (size_t)jb * N] : 0.0;
   }
+  const size_t cs = (size_t)chain * OBS_SLOTS + a.slot;  // (chain, slot)
   if (tid < OBS_TB) {
     const int i = i0 + tid;
     ni[tid] = i < N ? 1.0 - (double)G[i + (size_t)i * N] : 0.0;
+    if (bi == bj && i < N) a.diag[cs * N + i] = ni[tid];
   } else if (tid < 2 * OBS_TB) {
     const int j = j0 + tid - OBS_TB;
     nj[tid - OBS_TB] = j < N ? 1.0 - (double)G[j + (size_t)j * N] : 0.0;
+  }
+  if (blockIdx.x == 0) {
+    const int8_t *s = a.sig + (size_t)chain * a.T * a.lat.N2 + (size_t)(a.nt - 1) * a.lat.N2;
+    for (int i = tid; i < N; i += blockDim.x) {
+      const int *pl = a.lat.plaq + 4 * i;
+      a.flux[cs * N + i] = (double)((int)s[pl[0]] * (int)s[pl[1]] * (int)s[pl[2]] * (int)s[pl[3]]);
+    }
   }
   __syncthreads();
   double pair = 0.0, cdw = 0.0, cdws = 0.0;
@@ -81,53 +99,55 @@ __global__ void __launch_bounds__(256) k_obs_tiles(ObsArgs<R> a) {
   double sums[3] = {pair, cdw, cdws};
   block_reduce_sum_n<double, 3>(sums, red);
   if (tid == 0) {
-    double *p = a.part + ((size_t)chain * a.part_stride + blockIdx.x) * 3;
+    double *p = a.part + (cs * a.part_stride + blockIdx.x) * 3;
     p[0] = sums[0];
     p[1] = sums[1];
     p[2] = sums[2];
   }
 }
 
-// density and flux of the slice, the sum of the tile partials, and the
-// accumulation into the bin (one workgroup of 256 threads per chain)
-template <typename R>
-__global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<R> a) {
-  const int N = a.lat.N, N2 = a.lat.N2;
+// density and flux of slots 0 .. nslots-1 (slices in the order they were
+// measured), the sums of their tile partials, and the accumulation into the bin
+// slice by slice (one workgroup of 256 threads per chain): every slice's sums and
+// additions are those of one launch per slice
+__global__ void __launch_bounds__(256) k_obs_finish(ObsArgs<double> a, int nslots) {
+  const int N = a.lat.N;
   const int chain = blockIdx.x, ntile = ((N + OBS_TB - 1) / OBS_TB) * ((N + OBS_TB - 1) / OBS_TB);
-  const R *G = a.G + (size_t)chain * N * N;
-  const int8_t *s = a.sig + (size_t)chain * a.T * N2 + (size_t)(a.nt - 1) * N2;
   __shared__ double red[80];
-  double dens = 0.0, flux = 0.0;
-  for (int i = threadIdx.x; i < N; i += blockDim.x) {
-    const double nd = 1.0 - (double)G[i + (size_t)i * N];
-    dens += nd + nd;
-    const int *pl = a.lat.plaq + 4 * i;
-    flux += (double)((int)s[pl[0]] * (int)s[pl[1]] * (int)s[pl[2]] * (int)s[pl[3]]);
-  }
-  const double *p = a.part + (size_t)chain * a.part_stride * 3;
-  double pair = 0.0, cdw = 0.0, cdws = 0.0;
-  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
-    pair += p[3 * t];
-    cdw += p[3 * t + 1];
-    cdws += p[3 * t + 2];
-  }
-  double sums[5] = {dens, flux, pair, cdw, cdws};
-  block_reduce_sum_n<double, 5>(sums, red);
-  dens = sums[0];
-  flux = sums[1];
-  pair = sums[2];
-  cdw = sums[3];
-  cdws = sums[4];
-  if (threadIdx.x == 0) {
-    double *acc = a.acc + (size_t)chain * OBS_NACC;
-    const double ft = flux / (double)N;
-    acc[OBS_FLUX_SUM] += ft;
-    acc[OBS_FLUX_SQ] += ft * ft;
-    acc[OBS_PAIR] += pair;
-    acc[OBS_CDW] += cdw;
-    acc[OBS_CDW_SHIFT] += cdws;
-    acc[OBS_DENSITY] += dens / (double)N;
-    acc[OBS_NOBS] += 1.0;
+  for (int sl = 0; sl < nslots; ++sl) {
+    const size_t cs = (size_t)chain * OBS_SLOTS + sl;
+    const double *dg = a.diag + cs * N, *fx = a.flux + cs * N;
+    double dens = 0.0, flux = 0.0;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+      const double nd = dg[i];
+      dens += nd + nd;
+      flux += fx[i];
+    }
+    const double *p = a.part + cs * a.part_stride * 3;
+    double pair = 0.0, cdw = 0.0, cdws = 0.0;
+    for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+      pair += p[3 * t];
+      cdw += p[3 * t + 1];
+      cdws += p[3 * t + 2];
+    }
+    double sums[5] = {dens, flux, pair, cdw, cdws};
+    block_reduce_sum_n<double, 5>(sums, red);
+    dens = sums[0];
+    flux = sums[1];
+    pair = sums[2];
+    cdw = sums[3];
+    cdws = sums[4];
+    if (threadIdx.x == 0) {
+      double *acc = a.acc + (size_t)chain * OBS_NACC;
+      const double ft = flux / (double)N;
+      acc[OBS_FLUX_SUM] += ft;
+      acc[OBS_FLUX_SQ] += ft * ft;
+      acc[OBS_PAIR] += pair;
+      acc[OBS_CDW] += cdw;
+      acc[OBS_CDW_SHIFT] += cdws;
+      acc[OBS_DENSITY] += dens / (double)N;
+      acc[OBS_NOBS] += 1.0;
+    }
   }
 }
 

@@ -109,8 +109,10 @@ struct z2_ctx {
   unsigned long long *accc = nullptr;
   double *xmax = nullptr, *xmean = nullptr;
   double *obs_acc = nullptr, *obs_res = nullptr;
-  double *opart = nullptr;  // per chain: tile partial sums of the equal-time observables
+  double *opart = nullptr;  // per chain and slot: tile partial sums of the equal-time observables
+  double *odiag = nullptr, *oflux = nullptr;  // per chain and slot: 1 - G(i, i), plaquette products
   int opart_stride = 0;     // tiles per chain
+  int obs_pend = 0;         // slices measured whose sums k_obs_finish has not yet added (slots 0 ..)
   // scratch
   double *M = nullptr, *Q = nullptr, *X = nullptr, *Y = nullptr, *Gn = nullptr;
   double *tau = nullptr, *Dtmp = nullptr;
@@ -253,7 +255,9 @@ static void register_fields(z2_ctx *c) {
   reg_field(c, c->xmean, 1, 1, PT);
   reg_field(c, c->obs_acc, OBS_NACC, 1, P);
   reg_field(c, c->obs_res, RES_N, 1, P);
-  reg_field(c, c->opart, 3 * (size_t)c->opart_stride, 1, 0);
+  reg_field(c, c->opart, OBS_SLOTS * 3 * (size_t)c->opart_stride, 1, 0);
+  reg_field(c, c->odiag, OBS_SLOTS * N, 1, 0);
+  reg_field(c, c->oflux, OBS_SLOTS * N, 1, 0);
   for (double **m : {&c->M, &c->Q, &c->X, &c->Y, &c->Gn, &c->qrw}) reg_field(c, *m, nn, 1, 0);
   reg_field(c, c->tau, N, 1, 0);
   reg_field(c, c->Dtmp, N, 1, 0);
@@ -291,6 +295,7 @@ static void register_fields(z2_ctx *c) {
 // A contiguous range [c0, c0 + cnt) of storage slots seen by the launch code as
 // the whole context (every per-chain pointer moved by c0 chains), on `stream`.
 static int u_flush(z2_ctx *c);
+static int obs_flush(z2_ctx *c);
 
 struct ChainRange {
   z2_ctx *c;
@@ -518,7 +523,9 @@ extern "C" int z2_ctx_create(const z2_params *p, int32_t n_chains, const z2_chai
     {
       const int ntb = (N + OBS_TB - 1) / OBS_TB;
       c->opart_stride = ntb * ntb;
-      if ((rc = dalloc(c, &c->opart, B * 3 * (size_t)c->opart_stride))) break;
+      if ((rc = dalloc(c, &c->opart, B * OBS_SLOTS * 3 * (size_t)c->opart_stride))) break;
+      if ((rc = dalloc(c, &c->odiag, B * OBS_SLOTS * (size_t)N))) break;
+      if ((rc = dalloc(c, &c->oflux, B * OBS_SLOTS * (size_t)N))) break;
     }
     if ((rc = dalloc(c, &c->M, B * nn))) break;
     if ((rc = dalloc(c, &c->Q, B * nn))) break;
@@ -1794,6 +1801,7 @@ static int launch_u_multi(z2_ctx *c, R *U, int cols, int nt0, int ns, int step) 
 // apply the deferred propagator wraps (before anything reads or writes the
 // propagators or changes a field the deferred slices were made from)
 static int u_flush(z2_ctx *c) {
+  CHK(obs_flush(c));  // the observables' pending sums too: every caller is a point where they may be read
   if (!c->ud_n) return Z2_OK;
   const int ns = c->ud_n;
   c->ud_n = 0;
@@ -1875,14 +1883,13 @@ static int pre_prop(z2_ctx *c, int nt) {
 }
 
 // ---------------------------------------------------------------- observables
-extern "C" int z2_obs_reset(z2_ctx *c) {
-  if (!c) return set_err(Z2_EINVAL, "null ctx");
-  CHK(dzero(c, c->obs_acc, (size_t)c->nch * OBS_NACC));
-  return Z2_OK;
-}
-
+// The equal-time observables of a slice are measured when the sweep reaches it
+// (k_obs_tiles: the pair sums by tiles, the diagonal and the plaquette products
+// into a slot) and added to the bin later, by one k_obs_finish launch for all
+// slices pending (at the next stabilization, the end of the sweep, or before
+// anything reads or resets the bin), slice by slice in the order measured.
 template <typename R>
-static int obs_equal_launch(z2_ctx *c, const R *G, int nt) {
+static ObsArgs<R> obs_args(z2_ctx *c, const R *G, int nt) {
   ObsArgs<R> o;
   o.G = G;
   o.sig = c->sig;
@@ -1895,26 +1902,59 @@ static int obs_equal_launch(z2_ctx *c, const R *G, int nt) {
   o.lat = c->lat;
   o.part = c->opart;
   o.part_stride = c->opart_stride;
+  o.diag = c->odiag;
+  o.flux = c->oflux;
+  o.slot = 0;
+  return o;
+}
+
+static int obs_flush(z2_ctx *c) {
+  if (!c->obs_pend) return Z2_OK;
+  const int ns = c->obs_pend;
+  c->obs_pend = 0;
+  hipLaunchKernelGGL(k_obs_finish, dim3(c->nch), dim3(256), 0, c->stream, obs_args<double>(c, nullptr, 0), ns);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
+extern "C" int z2_obs_reset(z2_ctx *c) {
+  if (!c) return set_err(Z2_EINVAL, "null ctx");
+  CHK(obs_flush(c));
+  CHK(dzero(c, c->obs_acc, (size_t)c->nch * OBS_NACC));
+  return Z2_OK;
+}
+
+template <typename R>
+static int obs_tiles_launch(z2_ctx *c, const R *G, int nt) {
+  if (c->obs_pend == OBS_SLOTS) CHK(obs_flush(c));
+  ObsArgs<R> o = obs_args<R>(c, G, nt);
+  o.slot = c->obs_pend;
   const int ntb = (c->N + OBS_TB - 1) / OBS_TB;
   hipLaunchKernelGGL((k_obs_tiles<R>), dim3(ntb * ntb, c->nch), dim3(256), 0, c->stream, o);
   LAUNCH_CHECK();
-  hipLaunchKernelGGL((k_obs_finish<R>), dim3(c->nch), dim3(256), 0, c->stream, o);
-  LAUNCH_CHECK();
+  c->obs_pend++;
+  return Z2_OK;
+}
+
+// slice nt measured; its sums are added to the bin by the next obs_flush
+static int obs_equal_defer(z2_ctx *c, int nt) {
+  CHK(prof_begin(c, Z2_K_OBS));
+  if (c->mixed) CHK(obs_tiles_launch<float>(c, c->Gf, nt));
+  else CHK(obs_tiles_launch<double>(c, c->G, nt));
+  CHK(prof_end(c, Z2_K_OBS, 2.0 * (c->mixed ? 4.0 : 8.0) * c->N * (double)c->N * c->nch));
   return Z2_OK;
 }
 
 extern "C" int z2_obs_equal_calc(z2_ctx *c, int32_t nt) {
   if (!c || nt < 1 || nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", nt);
-  CHK(prof_begin(c, Z2_K_OBS));
-  if (c->mixed) CHK(obs_equal_launch<float>(c, c->Gf, nt));
-  else CHK(obs_equal_launch<double>(c, c->G, nt));
-  CHK(prof_end(c, Z2_K_OBS, 2.0 * (c->mixed ? 4.0 : 8.0) * c->N * (double)c->N * c->nch));
-  return Z2_OK;
+  CHK(obs_equal_defer(c, nt));
+  return obs_flush(c);
 }
 
 // Obs_equal%ave (src/obser_equal.f90:50-71); nobs > 0 is the caller's sample
 // count, else the count of z2_obs_equal_calc calls since the reset
 static int obs_ave(z2_ctx *c, int nobs = 0) {
+  CHK(obs_flush(c));
   hipLaunchKernelGGL(k_obs_ave, dim3((c->nch + 63) / 64), dim3(64), 0, c->stream, (const double *)c->obs_acc,
                      c->obs_res, (const ChainConst *)c->dcc, c->N, c->nch, nobs);
   LAUNCH_CHECK();
@@ -2394,7 +2434,9 @@ static int run_step(z2_ctx *c, const Step &st) {
   switch (st.kind) {
     case ST_WRAP_L: return z2_wrap_L(c, st.nt, st.flag);
     case ST_WRAP_R: return z2_wrap_R(c, st.nt, st.flag);
-    case ST_OBS: return z2_obs_equal_calc(c, st.nt);
+    case ST_OBS:
+      if (st.nt < 1 || st.nt > c->T) return set_err(Z2_EINVAL, "bad slice %d", st.nt);
+      return obs_equal_defer(c, st.nt);
     case ST_SLICE_L: return z2_slice_L(c, st.nt);
     case ST_SLICE_R: return z2_slice_R(c, st.nt);
     case ST_DYN: return dyn_sweep(c);
