@@ -653,8 +653,14 @@ __device__ __forceinline__ double trsm_row_bcast(double v, int kk) {
   }
 }
 
+// dsc (chain stride N), if given: the right-hand side's solve index i divided by
+// dsc[i] as it is read (the D^{-1} of the G recompute); perm (chain stride N,
+// 1-based), if given: the solution written to Y at solve index perm[i] - 1 instead
+// of in place (the P of the G recompute) -- the arithmetic of a division kernel
+// before and a permutation kernel after the solve
 template <int MJ>
-__global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, int N, int rs, int cs, int mode) {
+__global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, int N, int rs, int cs, int mode,
+                                                  const double *dsc, const int *perm, double *Y) {
   extern __shared__ unsigned char smem_raw[];
   double *rt = reinterpret_cast<double *>(smem_raw);  // [16][N]: columns 16 jk + kk of R
   const int chain = blockIdx.y;
@@ -668,6 +674,14 @@ __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, i
   for (int j = 0; j < MJ; ++j) {
     const int i = l16 + 16 * j;
     xr[j] = (cin && i < N) ? Xc[(size_t)i * rs + (size_t)c * cs] : 0.0;
+  }
+  if (dsc) {
+    const double *dc = dsc + (size_t)chain * N;
+#pragma unroll
+    for (int j = 0; j < MJ; ++j) {
+      const int i = l16 + 16 * j;
+      if (cin && i < N) xr[j] = xr[j] / dc[i];
+    }
   }
   // N <= 256 (one row of R per thread): the next block's 16 columns are read into
   // registers while this block is solved, and stored to LDS at the next step
@@ -740,10 +754,12 @@ __global__ void __launch_bounds__(256) k_trsm_reg(const double *Rm, double *X, i
       xr[j] = x;
     }
   }
+  const int *pc = perm ? perm + (size_t)chain * N : nullptr;
+  double *Yc = perm ? Y + (size_t)chain * N * N : Xc;
 #pragma unroll
   for (int j = 0; j < MJ; ++j) {
     const int i = l16 + 16 * j;
-    if (cin && i < N) Xc[(size_t)i * rs + (size_t)c * cs] = xr[j];
+    if (cin && i < N) Yc[(size_t)(pc ? pc[i] - 1 : i) * rs + (size_t)c * cs] = xr[j];
   }
 }
 

@@ -1514,13 +1514,14 @@ static int apply_qt(z2_ctx *c, const double *A, const double *B, double *X, int 
 // n x n triangular solves for nb matrices; the right-hand-side block shrinks so that
 // (cbt + 1) n doubles fit the LDS of one workgroup
 template <int MJ>
-static int launch_trsm_reg(int n, int nb, hipStream_t st, const double *R, double *X, int right) {
+static int launch_trsm_reg(int n, int nb, hipStream_t st, const double *R, double *X, int right,
+                           const double *dsc = nullptr, const int *perm = nullptr, double *Y = nullptr) {
   const size_t sh = sizeof(double) * 16 * (size_t)n;
   if (sh > 65536)
     HIPC(hipFuncSetAttribute((const void *)k_trsm_reg<MJ>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   dim3 grid((n + 15) / 16, nb);
-  if (right) hipLaunchKernelGGL(k_trsm_reg<MJ>, grid, dim3(256), sh, st, R, X, n, n, 1, 1);
-  else hipLaunchKernelGGL(k_trsm_reg<MJ>, grid, dim3(256), sh, st, R, X, n, 1, n, 0);
+  if (right) hipLaunchKernelGGL(k_trsm_reg<MJ>, grid, dim3(256), sh, st, R, X, n, n, 1, 1, dsc, perm, Y);
+  else hipLaunchKernelGGL(k_trsm_reg<MJ>, grid, dim3(256), sh, st, R, X, n, 1, n, 0, dsc, perm, Y);
   LAUNCH_CHECK();
   return Z2_OK;
 }
@@ -1552,6 +1553,27 @@ static int trsm(z2_ctx *c, const double *R, double *X, int right, int n = 0) {
   CHK(launch_trsm(m, c->nch, c->stream, c->lds_max, R, X, right));
   CHK(prof_end(c, Z2_K_TRSM, (double)m * m * (double)m * c->nch));
   return Z2_OK;
+}
+
+// Y = P R^{-1} D^{-1} X (right = 0) or Y = X D^{-1} R^{-1} P^T (right = 1), N x N:
+// the row (column) division by D, the triangular solve and the row (column)
+// permutation, in one launch where the register solve applies (X is left
+// unchanged then), else the three launches it fuses
+static int trsm_dp(z2_ctx *c, const double *R, double *X, int right, const double *D, const int *perm, double *Y) {
+  const int n = c->N;
+  if (sizeof(double) * 16 * (size_t)n <= c->lds_max && n <= 1024) {
+    CHK(prof_begin(c, Z2_K_TRSM));
+    if (n <= 64) CHK(launch_trsm_reg<4>(n, c->nch, c->stream, R, X, right, D, perm, Y));
+    else if (n <= 128) CHK(launch_trsm_reg<8>(n, c->nch, c->stream, R, X, right, D, perm, Y));
+    else if (n <= 256) CHK(launch_trsm_reg<16>(n, c->nch, c->stream, R, X, right, D, perm, Y));
+    else if (n <= 512) CHK(launch_trsm_reg<32>(n, c->nch, c->stream, R, X, right, D, perm, Y));
+    else CHK(launch_trsm_reg<64>(n, c->nch, c->stream, R, X, right, D, perm, Y));
+    CHK(prof_end(c, Z2_K_TRSM, (double)n * n * (double)n * c->nch));
+    return Z2_OK;
+  }
+  CHK(elem(c, X, X, right ? 9 : 8, D));
+  CHK(trsm(c, R, X, right));
+  return elem(c, X, Y, right ? 7 : 6, nullptr, perm);
 }
 
 // device copies as a kernel (every step of a sweep stays a kernel launch, so that
@@ -1614,9 +1636,7 @@ static int stab_green_split(z2_ctx *c) {
   // DRb^{-1} UR^T = (UR DRb^{-1})^T: divide the columns of UR by DRb
   CHK(elem_d(c, c->UUR, c->Gn, 9, c->dsc, 4));
   CHK(gemm(c, c->Q, 1, c->Gn, 1, c->X));           // Q^T DRb^{-1} UR^T
-  CHK(elem(c, c->X, c->X, 8, c->Dtmp));            // D^{-1} ...
-  CHK(trsm(c, c->M, c->X, 0));                     // V^{-1} ...
-  CHK(elem(c, c->X, c->Y, 6, nullptr, c->ipvt));   // P ...
+  CHK(trsm_dp(c, c->M, c->X, 0, c->Dtmp, c->ipvt, c->Y));  // P V^{-1} D^{-1} ...
   CHK(elem_d(c, c->Y, c->Y, 8, c->dsc + 2 * (size_t)c->N, 4));  // DLb^{-1} ...
   CHK(gemm(c, c->UUL, 1, c->Y, 0, c->Gn));         // UL^T ...
   return Z2_OK;
@@ -1637,15 +1657,11 @@ static int stab_green(z2_ctx *c, int nt) {
   CHK(qrp(c, c->M, c->Dtmp));
   if (nt < c->T / 2 + 1) {
     CHK(apply_qt(c, c->M, c->UUR, c->X, 0));  // X = Q^T UUR^T
-    CHK(elem(c, c->X, c->X, 8, c->Dtmp));
-    CHK(trsm(c, c->M, c->X, 0));
-    CHK(elem(c, c->X, c->Y, 6, nullptr, c->ipvt));
+    CHK(trsm_dp(c, c->M, c->X, 0, c->Dtmp, c->ipvt, c->Y));
     CHK(gemm(c, c->UUL, 1, c->Y, 0, c->Gn));
   } else {
     CHK(apply_qt(c, c->M, c->UUL, c->X, 1));  // X = UUL^T Q
-    CHK(elem(c, c->X, c->X, 9, c->Dtmp));
-    CHK(trsm(c, c->M, c->X, 1));
-    CHK(elem(c, c->X, c->Y, 7, nullptr, c->ipvt));
+    CHK(trsm_dp(c, c->M, c->X, 1, c->Dtmp, c->ipvt, c->Y));
     CHK(gemm(c, c->Y, 0, c->UUR, 1, c->Gn));
   }
   return Z2_OK;
@@ -1995,9 +2011,7 @@ static int ginv(z2_ctx *c, const double *A, double *Ainv) {
   CHK(qrp(c, c->M, c->Dtmp));
   CHK(orgqr(c, c->M, c->Q));
   CHK(elem(c, c->Q, c->X, 1));
-  CHK(elem(c, c->X, c->X, 8, c->Dtmp));
-  CHK(trsm(c, c->M, c->X, 0));
-  CHK(elem(c, c->X, Ainv, 6, nullptr, c->ipvt));
+  CHK(trsm_dp(c, c->M, c->X, 0, c->Dtmp, c->ipvt, Ainv));
   return Z2_OK;
 }
 
