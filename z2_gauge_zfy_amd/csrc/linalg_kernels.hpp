@@ -505,10 +505,14 @@ __device__ __forceinline__ double gemm_ld(const double *X, int ld, int trans, in
   return keep ? v : 0.0;
 }
 
-__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+// Two independent products of the same shape in one launch: grid.z = nfirst chains
+// of g0, then the chains of g1 (g1 = g0 and grid.z = nfirst for one product).
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g0, GemmArgs g1, int nfirst) {
   __shared__ double As[GK][GT + 1];  // As[k][m]
   __shared__ double Bs[GK][GT + 1];  // Bs[k][n]
-  const int chain = blockIdx.z;
+  const bool second = (int)blockIdx.z >= nfirst;
+  const GemmArgs g = second ? g1 : g0;
+  const int chain = (int)blockIdx.z - (second ? nfirst : 0);
   const double *A = g.A + chain * g.sA;
   const double *B = g.B + chain * g.sB;
   double *C = g.C + chain * g.sC;

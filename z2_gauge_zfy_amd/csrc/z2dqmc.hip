@@ -1199,9 +1199,8 @@ static int elem(z2_ctx *c, const double *in, double *out, int mode, const double
 }
 
 // general batched GEMM: C(M x Nc) = alpha op(A) op(B) + beta C, chain strides sA/sB/sC
-static int gemm_g(z2_ctx *c, int M, int Nc, int K, const double *A, int lda, long sA, int ta,
-                  const double *B, int ldb, long sB, int tb, double *C, int ldc, long sC, double alpha = 1.0,
-                  double beta = 0.0, int triA = 0, int triB = 0) {
+static GemmArgs gemm_args(int M, int Nc, int K, const double *A, int lda, long sA, int ta, const double *B, int ldb,
+                          long sB, int tb, double *C, int ldc, long sC, double alpha, double beta, int triA, int triB) {
   GemmArgs g;
   g.A = A;
   g.B = B;
@@ -1221,11 +1220,33 @@ static int gemm_g(z2_ctx *c, int M, int Nc, int K, const double *A, int lda, lon
   g.triB = triB;
   g.alpha = alpha;
   g.beta = beta;
+  return g;
+}
+
+static int gemm_g(z2_ctx *c, int M, int Nc, int K, const double *A, int lda, long sA, int ta,
+                  const double *B, int ldb, long sB, int tb, double *C, int ldc, long sC, double alpha = 1.0,
+                  double beta = 0.0, int triA = 0, int triB = 0) {
+  const GemmArgs g = gemm_args(M, Nc, K, A, lda, sA, ta, B, ldb, sB, tb, C, ldc, sC, alpha, beta, triA, triB);
   dim3 grid((M + GT - 1) / GT, (Nc + GT - 1) / GT, c->nch);
   CHK(prof_begin(c, Z2_K_GEMM));
-  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, c->stream, g);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, c->stream, g, g, c->nch);
   LAUNCH_CHECK();
   CHK(prof_end(c, Z2_K_GEMM, 2.0 * M * Nc * (double)K * c->nch));
+  return Z2_OK;
+}
+
+// two independent square N x N GEMMs on per-chain matrices in one launch
+static int gemm2(z2_ctx *c, const double *A0, int ta0, const double *B0, int tb0, double *C0, const double *A1, int ta1,
+                 const double *B1, int tb1, double *C1) {
+  const int N = c->N;
+  const long nn = (long)N * N;
+  const GemmArgs g0 = gemm_args(N, N, N, A0, N, nn, ta0, B0, N, nn, tb0, C0, N, nn, 1.0, 0.0, 0, 0);
+  const GemmArgs g1 = gemm_args(N, N, N, A1, N, nn, ta1, B1, N, nn, tb1, C1, N, nn, 1.0, 0.0, 0, 0);
+  dim3 grid((N + GT - 1) / GT, (N + GT - 1) / GT, 2 * c->nch);
+  CHK(prof_begin(c, Z2_K_GEMM));
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, c->stream, g0, g1, c->nch);
+  LAUNCH_CHECK();
+  CHK(prof_end(c, Z2_K_GEMM, 4.0 * N * N * (double)N * c->nch));
   return Z2_OK;
 }
 
@@ -1591,6 +1612,32 @@ static int dcopy(z2_ctx *c, double *dst, const double *src, size_t n) {
   return Z2_OK;
 }
 
+// three copies in one launch (a UDV triple to or from its checkpoint slot): index
+// i of the concatenation of the three ranges
+struct CopySeg {
+  double *dst;
+  const double *src;
+  size_t n;
+};
+
+__global__ void __launch_bounds__(256) k_dcopy3(CopySeg a, CopySeg b, CopySeg d) {
+  const size_t ab = a.n + b.n, total = ab + d.n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+    if (i < a.n) a.dst[i] = a.src[i];
+    else if (i < ab) b.dst[i - a.n] = b.src[i - a.n];
+    else d.dst[i - ab] = d.src[i - ab];
+  }
+}
+
+static int dcopy3(z2_ctx *c, CopySeg a, CopySeg b, CopySeg d) {
+  const size_t n = a.n + b.n + d.n;
+  if (!n) return Z2_OK;
+  size_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL(k_dcopy3, dim3((unsigned)(nb > 2048 ? 2048 : nb)), dim3(256), 0, c->stream, a, b, d);
+  LAUNCH_CHECK();
+  return Z2_OK;
+}
+
 // ---------------------------------------------------------------- stabilization
 // stab_UR, src/stabilization.f90:70-94, on the right propagator (U, D, V)
 static int stab_UR_buf(z2_ctx *c, double *U, double *D, double *V) {
@@ -1644,8 +1691,7 @@ static int stab_green_split(z2_ctx *c) {
 
 static int stab_green(z2_ctx *c, int nt) {
   if (c->stab_mode == Z2_STAB_DSPLIT) return stab_green_split(c);
-  CHK(gemm(c, c->VUR, 0, c->VUL, 0, c->Y));
-  CHK(gemm(c, c->UUR, 1, c->UUL, 1, c->X));
+  CHK(gemm2(c, c->VUR, 0, c->VUL, 0, c->Y, c->UUR, 1, c->UUL, 1, c->X));  // Y = VR VL, X = UR^T UL^T
   const int trans = nt > c->T / 2;
   {
     int nb = (c->N * c->N + 255) / 256;
@@ -1696,9 +1742,8 @@ extern "C" int z2_wrap_pre(z2_ctx *c, int32_t nt) {
     if (c->mixed && !c->u64) CHK(cvt(c, (const float *)c->UURf, c->UUR));
     CHK(stab_UR(c));
   }
-  CHK(dcopy(c, c->URl + slot(c, st), c->UUR, nn));
-  CHK(dcopy(c, c->VRl + slot(c, st), c->VUR, nn));
-  CHK(dcopy(c, c->DRl + slotd(c, st), c->DUR, nd));
+  CHK(dcopy3(c, {c->URl + slot(c, st), c->UUR, nn}, {c->VRl + slot(c, st), c->VUR, nn},
+             {c->DRl + slotd(c, st), c->DUR, nd}));
   if (nt == c->T) {
     CHK(stab_green(c, nt));
     if (c->mixed) CHK(cvt(c, (const double *)c->Gn, c->Gf));
@@ -1717,18 +1762,16 @@ extern "C" int z2_wrap_L(z2_ctx *c, int32_t nt, int32_t flag) {
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect ortholeft time slice %d", nt);
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
-  CHK(dcopy(c, c->UUR, c->URl + slot(c, st), nn));
-  CHK(dcopy(c, c->VUR, c->VRl + slot(c, st), nn));
-  CHK(dcopy(c, c->DUR, c->DRl + slotd(c, st), nd));
+  CHK(dcopy3(c, {c->UUR, c->URl + slot(c, st), nn}, {c->VUR, c->VRl + slot(c, st), nn},
+             {c->DUR, c->DRl + slotd(c, st), nd}));
   if (nt != c->T) {
     if (c->mixed && !c->u64) CHK(cvt(c, (const float *)c->UULf, c->UUL));
     CHK(stab_UL(c));
     CHK(stab_green(c, nt));
     CHK(drift_replace(c, flag));
   }
-  CHK(dcopy(c, c->ULl + slot(c, st), c->UUL, nn));
-  CHK(dcopy(c, c->VLl + slot(c, st), c->VUL, nn));
-  CHK(dcopy(c, c->DLl + slotd(c, st), c->DUL, nd));
+  CHK(dcopy3(c, {c->ULl + slot(c, st), c->UUL, nn}, {c->VLl + slot(c, st), c->VUL, nn},
+             {c->DLl + slotd(c, st), c->DUL, nd}));
   if (c->mixed && !c->u64) CHK(cvt(c, (const double *)c->UUL, c->UULf));
   if (nt == 0) c->left_done = c->nch == c->nch_all ? 1 : -1;
   return Z2_OK;
@@ -1739,18 +1782,16 @@ extern "C" int z2_wrap_R(z2_ctx *c, int32_t nt, int32_t flag) {
   if (!c || nt < 0 || nt > c->T || nt % c->w) return set_err(Z2_EINVAL, "incorrect orthoright time slice %d", nt);
   const int st = nt / c->w;
   const size_t nn = (size_t)c->nch * c->N * c->N, nd = (size_t)c->nch * c->N;
-  CHK(dcopy(c, c->UUL, c->ULl + slot(c, st), nn));
-  CHK(dcopy(c, c->VUL, c->VLl + slot(c, st), nn));
-  CHK(dcopy(c, c->DUL, c->DLl + slotd(c, st), nd));
+  CHK(dcopy3(c, {c->UUL, c->ULl + slot(c, st), nn}, {c->VUL, c->VLl + slot(c, st), nn},
+             {c->DUL, c->DLl + slotd(c, st), nd}));
   if (nt != 0) {
     if (c->mixed && !c->u64) CHK(cvt(c, (const float *)c->UURf, c->UUR));
     CHK(stab_UR(c));
     CHK(stab_green(c, nt));
     CHK(drift_replace(c, flag));
   }
-  CHK(dcopy(c, c->URl + slot(c, st), c->UUR, nn));
-  CHK(dcopy(c, c->VRl + slot(c, st), c->VUR, nn));
-  CHK(dcopy(c, c->DRl + slotd(c, st), c->DUR, nd));
+  CHK(dcopy3(c, {c->URl + slot(c, st), c->UUR, nn}, {c->VRl + slot(c, st), c->VUR, nn},
+             {c->DRl + slotd(c, st), c->DUR, nd}));
   if (c->mixed && !c->u64) CHK(cvt(c, (const double *)c->UUR, c->UURf));
   if (nt == c->T) c->left_done = c->nch == c->nch_all ? 0 : -1;
   return Z2_OK;
@@ -2991,7 +3032,7 @@ extern "C" int z2_test_gemm(int32_t n, const double *A, int32_t ta, const double
   g.lda = g.ldb = g.ldc = n;
   g.ta = ta; g.tb = tb; g.triA = triA; g.triB = triB; g.alpha = 1.0; g.beta = 0.0;
   dim3 grid((n + GT - 1) / GT, (n + GT - 1) / GT, 1);
-  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, 0, g);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, 0, g, g, 1);
   LAUNCH_CHECK();
   HIPC(hipDeviceSynchronize());
   HIPC(hipMemcpy(C, dC, nn * 8, hipMemcpyDeviceToHost));
