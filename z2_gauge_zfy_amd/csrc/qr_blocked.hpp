@@ -33,6 +33,7 @@ struct QrbArgs {
   double *Xq;         // Q^T B: output X (chain stride N*N), written as X^T when xtrans
   int btrans, xtrans;
   int N, tc, nopiv;
+  int p0w;  // k_qr_trail: panel 0 is in columns 0 .. NB-1 of W (k_qr_panel0), not yet in A
 };
 
 typedef double qrb_d4 __attribute__((ext_vector_type(4)));
@@ -865,19 +866,20 @@ __global__ void __launch_bounds__(512) k_qr_order(QrbArgs a) {
 }
 
 // Panel p of one matrix: factored from the m x w block at Wp (column stride ldW,
-// global or LDS) into L.Pn; writes its columns of A (normalised R rows of the
-// diagonal block, signs, reflectors), tau, D and its T into Tws.
+// global or LDS; column c at Wp + cperm[c] ldW if cperm) into L.Pn; writes its
+// columns of A (normalised R rows of the diagonal block, signs, reflectors; of
+// Out instead when given), tau, D and its T into Tws.
 template <int NB, int RPL>
 __device__ void qrb_panel_step(const QrbArgs &a, QrbLds &L, double *Tws, int chain, int p, const double *Wp,
-                               int ldW) {
+                               int ldW, const int *cperm = nullptr, double *Out = nullptr) {
   const int N = a.N;
   const int npan = (N + NB - 1) / NB;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  double *A = a.A + (size_t)chain * N * N;
+  double *A = Out ? Out : a.A + (size_t)chain * N * N;
   double *tau = a.tau + (size_t)chain * N;
   double *D = a.D + (size_t)chain * N;
   const int j0 = p * NB, w = min(NB, N - j0), m = N - j0;
-  qrb_panel_reg<NB, RPL>(L, Wp, ldW, nullptr, m, w, N - 1 - j0);
+  qrb_panel_reg<NB, RPL>(L, Wp, ldW, cperm, m, w, N - 1 - j0);
   // T of every panel, the last one too: the explicit Q and the Q^T application
   // that follow the factorization use them as they are
   qrb_build_t<NB>(L, m, w);
@@ -910,6 +912,31 @@ __global__ void __launch_bounds__(512) k_qr_panel(QrbArgs a, double *Tws, int p)
   qrb_panel_step<NB, RPL>(a, L, Tws, chain, p, a.W + (size_t)chain * N * N + j0 + (size_t)j0 * N, N);
 }
 
+// Panel 0 and the column order in one launch (N > NB): workgroup (0, chain)
+// factors panel 0 from A through the order of k_qr_order (ipvt) into columns
+// 0 .. NB-1 of W, the other workgroups of the chain copy the ordered columns NB ..
+// N-1 of A into W (W = A P beyond the panel).  A is only read here, so it may be
+// written from the next launch on, which takes panel 0 from W and copies it to A.
+template <int NB, int RPL>
+__global__ void __launch_bounds__(512) k_qr_panel0(QrbArgs a, double *Tws) {
+  const int N = a.N, chain = blockIdx.y;
+  const int *ipvt = a.ipvt + (size_t)chain * N;
+  const double *A = a.A + (size_t)chain * N * N;
+  double *W = a.W + (size_t)chain * N * N;
+  if (blockIdx.x == 0) {
+    extern __shared__ unsigned char smem_raw[];
+    QrbLds L = qrb_carve(smem_raw, N, NB, a.tc);
+    for (int c = threadIdx.x; c < NB; c += blockDim.x) L.perm[c] = ipvt[c] - 1;
+    __syncthreads();
+    qrb_panel_step<NB, RPL>(a, L, Tws, chain, 0, A, N, L.perm, W);
+    return;
+  }
+  for (int c = NB + (int)blockIdx.x - 1; c < N; c += (int)gridDim.x - 1) {
+    const double *src = A + (size_t)(ipvt[c] - 1) * N;
+    for (int r = threadIdx.x; r < N; r += blockDim.x) W[r + (size_t)c * N] = src[r];
+  }
+}
+
 // lookahead (tile width == NB): the workgroup of tile 0, whose columns are the
 // next panel's, goes on to factor panel p + 1 from its updated tile in LDS while
 // the other tiles are updated (the next launch is then panel p + 1's trail).
@@ -927,11 +954,20 @@ __global__ void __launch_bounds__(512) k_qr_trail(QrbArgs a, double *Tws, int p,
   const double *D = a.D + (size_t)chain * N;
   extern __shared__ unsigned char smem_raw[];
   QrbLds L = qrb_carve(smem_raw, N, NB, tc);
-  qrb_tile_load<RPL>(L.Pn, L.ld, A + j0 + (size_t)j0 * N, N, nullptr, m, w);  // reflectors below the diagonal
+  // reflectors below the diagonal (panel 0: from W, where k_qr_panel0 left it)
+  const double *Vp = p == 0 && a.p0w ? W : A + j0 + (size_t)j0 * N;
+  qrb_tile_load<RPL>(L.Pn, L.ld, Vp, N, nullptr, m, w);
   const double *T = Tws + ((size_t)chain * npan + p) * NB * NB;
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) L.Tm[(e % NB) + (size_t)(e / NB) * L.lw] = T[e];
   qrb_tile_load<RPL>(L.Ct, L.ld, W + j0 + (size_t)c0 * N, N, nullptr, m, tcn);
   __syncthreads();
+  if (p == 0 && a.p0w && blockIdx.x == 0)  // panel 0's columns of A (m = N rows)
+    for (int c = wave; c < w; c += nw)
+#pragma unroll
+      for (int t = 0; t < RPL; ++t) {
+        const int r = lane + 64 * t;
+        if (r < m) A[r + (size_t)c * N] = L.Pn[(size_t)c * L.ld + r];
+      }
   qrb_apply_tile<NB>(L, m, w, tcn, 1);
   for (int c = wave; c < tcn; c += nw)
 #pragma unroll

@@ -1375,27 +1375,40 @@ static int launch_orgqr_split_nb(int rpl, int n, int tc, int nmat, size_t sh, hi
   return set_err(Z2_EINVAL, "blocked QR supports N <= 512");
 }
 
-// the QR over many workgroups per matrix (order, copy of the ordered columns to
-// W, then per panel: factorization, trailing update)
+// workgroups per chain that copy the ordered columns beside panel 0
+constexpr int QR_GATHER_WG = 8;
+
+// the QR over many workgroups per matrix (order; panel 0 with the copy of the
+// ordered columns to W; then per panel: trailing update, next panel)
 template <int NB, int RPL>
 static int launch_qr_split_t(z2_ctx *c, size_t sh, const QrbArgs &q) {
   const int n = q.N, tc = q.tc, npan = (n + NB - 1) / NB;
   if (sh > 65536) {
     HIPC(hipFuncSetAttribute((const void *)k_qr_panel<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+    HIPC(hipFuncSetAttribute((const void *)k_qr_panel0<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
     HIPC(hipFuncSetAttribute((const void *)k_qr_trail<NB, RPL>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
   }
   hipLaunchKernelGGL((k_qr_order<RPL>), dim3(c->nch), dim3(512), sizeof(double) * n, c->stream, q);
   LAUNCH_CHECK();
-  CHK(elem(c, q.A, q.W, 5, nullptr, q.ipvt));  // W = A P
   const int lookahead = tc == NB;  // tile 0 holds exactly the next panel's columns
-  hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, 0);
-  LAUNCH_CHECK();
+  QrbArgs qw = q;
+  if (n > NB) {
+    // panel 0 into W with W = A P beyond it, one launch; the first trail launch
+    // copies panel 0 to A
+    qw.p0w = 1;
+    hipLaunchKernelGGL((k_qr_panel0<NB, RPL>), dim3(1 + QR_GATHER_WG, c->nch), dim3(512), sh, c->stream, qw, c->ortw);
+    LAUNCH_CHECK();
+  } else {
+    CHK(elem(c, q.A, q.W, 5, nullptr, q.ipvt));  // W = A P
+    hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, 0);
+    LAUNCH_CHECK();
+  }
   for (int p = 0; p < npan; ++p) {
     const int j0 = p * NB, w = n - j0 < NB ? n - j0 : NB;
     if (j0 + w >= n) break;
     const int ntile = (n - j0 - w + tc - 1) / tc;
-    hipLaunchKernelGGL((k_qr_trail<NB, RPL>), dim3(ntile, c->nch), dim3(512), sh, c->stream, q, c->ortw, p,
-                       lookahead);
+    hipLaunchKernelGGL((k_qr_trail<NB, RPL>), dim3(ntile, c->nch), dim3(512), sh, c->stream, p == 0 ? qw : q, c->ortw,
+                       p, lookahead);
     LAUNCH_CHECK();
     if (!lookahead) {
       hipLaunchKernelGGL((k_qr_panel<NB, RPL>), dim3(c->nch), dim3(512), sh, c->stream, q, c->ortw, p + 1);
