@@ -87,10 +87,12 @@ def test_parity_batched_uneven_couplings_and_mu():
     _run_parity(6, 20, 2.0, 5, specs, nsweeps=3)
 
 
-@pytest.mark.parametrize("L", [16, 20])
+@pytest.mark.parametrize("L", [16, 18, 20])
 def test_parity_larger_lattices(L):
     """The benchmark's lattice, L=16 (N=256: one proposal per flip thread, QR
-    panels of 32 columns split over many workgroups), and L=20 (N=400: two
+    panels of 32 columns split over many workgroups, the next panel factored in
+    the trailing update's launch), L=18 (N=324: panels of 32 columns with 16-column
+    trailing tiles, each panel in a launch of its own) and L=20 (N=400: two
     proposals per flip thread, QR panels of 16 columns), Ltrot=20, beta=2,
     Nwrap=10, two chains (mu = 0 and 0.2), one full sweep with its recomputes."""
     import z2_gauge_zfy_amd as z
