@@ -809,9 +809,11 @@ __global__ void __launch_bounds__(512) k_orgqr_tiles(QrbArgs a, const double *Tw
 }
 
 // The QR spread over many workgroups per matrix.  k_qr_order: column norms, the
-// order (decreasing norm, ties by index) into ipvt; the host then copies the
-// ordered columns into W.  Per panel, k_qr_panel (one workgroup per matrix)
-// factors the panel from W, writes its columns of A (normalised R rows of the
+// order (decreasing norm, ties by index) into ipvt.  k_qr_panel0 factors panel 0
+// through that order while its other workgroups copy the remaining ordered
+// columns into W (for N <= NB: the copy, then k_qr_panel).  Per later panel,
+// k_qr_panel (one workgroup per matrix) factors the panel from W (or the
+// look-ahead below does), writes its columns of A (normalised R rows of the
 // diagonal block, signs, reflectors), tau, D and the block reflector's T into
 // Tws; k_qr_trail applies it to the trailing columns with one workgroup per
 // column tile: the panel's rows become final normalised R rows in A, the rest go
